@@ -1,0 +1,165 @@
+"""Drop-in replacement of the reference's ``ct_laboratory._C`` extension.
+
+Same 13 function names, positional arguments, output shapes and error type as
+src/bindings.cpp:65-104, implemented over the C ABI of libctlab_hip.so.
+Differences from the reference, all deliberate:
+
+* launches go to torch's *current* stream (the reference uses the legacy
+  default stream, src/ct_projector_3d.cu:425) and every launch is checked;
+* sizes are 64-bit (the reference truncates to ``int``, src/ct_projector_2d.cu:54);
+* dtype, device and contiguity are checked (RuntimeError), where the reference
+  only checks ``is_cuda``.
+
+Outputs are allocated here with torch (as the reference's host wrappers do)
+and carry a leading batch dimension.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _native as N
+
+
+def _require(cond: bool, msg: str) -> None:
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _check_f32(name: str, *tensors: torch.Tensor) -> None:
+    for t in tensors:
+        _require(isinstance(t, torch.Tensor), f"{name}: expected a tensor")
+        _require(t.is_cuda, f"{name}: inputs must be CUDA (HIP) tensors")
+        _require(t.dtype == torch.float32, f"{name}: expected float32, got {t.dtype}")
+        _require(t.is_contiguous(), f"{name}: expected a contiguous tensor")
+    dev = tensors[0].device
+    for t in tensors[1:]:
+        _require(t.device == dev, f"{name}: all tensors must be on one device")
+
+
+def _rays(name: str, src: torch.Tensor, dst: torch.Tensor, nd: int) -> int:
+    _require(src.dim() == 2 and src.shape[1] == nd, f"{name}: src must be [n_ray, {nd}]")
+    _require(dst.shape == src.shape, f"{name}: dst must match src")
+    return int(src.shape[0])
+
+
+def _affine(name: str, M_inv: torch.Tensor, b: torch.Tensor, nd: int) -> None:
+    _require(M_inv.numel() == nd * nd, f"{name}: M_inv must have {nd}x{nd} elements")
+    _require(b.numel() == nd, f"{name}: b must have {nd} elements")
+
+
+def _vol_dims(name: str, vol: torch.Tensor, nd: int):
+    if vol.dim() == nd:
+        return 1, tuple(int(s) for s in vol.shape)
+    if vol.dim() == nd + 1:
+        return int(vol.shape[0]), tuple(int(s) for s in vol.shape[1:])
+    raise RuntimeError(f"{name}: volume shape!" if nd == 3 else f"{name}: image shape!")
+
+
+def _sino_dims(name: str, sino: torch.Tensor):
+    if sino.dim() == 1:
+        return 1, int(sino.shape[0])
+    if sino.dim() == 2:
+        return int(sino.shape[0]), int(sino.shape[1])
+    raise RuntimeError(f"{name}: sino shape!")
+
+
+# ---------------------------------------------------------------------------
+# 3D on-the-fly (bindings.cpp:98-103)
+# ---------------------------------------------------------------------------
+
+
+def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF):
+    """[B, n_ray] line integrals (src/ct_projector_3d.cu:1018-1049)."""
+    name = "forward_project_3d_on_the_fly_cuda"
+    _check_f32(name, volume, src, dst, M_inv, b)
+    batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
+    n_ray = _rays(name, src, dst, 3)
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    with torch.cuda.device(volume.device):
+        N.call("ctlab_forward_project_3d_otf", N.ptr(volume), batch, nx, ny, nz, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), mode, N.ptr(out), N.stream_of(volume))
+    return out
+
+
+def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF):
+    """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079)."""
+    name = "back_project_3d_on_the_fly_cuda"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_otf", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
+               N.ptr(b), int(n_x), int(n_y), int(n_z), mode, 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 2D on-the-fly (bindings.cpp:75-80)
+# ---------------------------------------------------------------------------
+
+
+def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N.MODE_OTF,
+                                       zero_rule: int = N.ZERO_THRESH):
+    """[B, n_ray] line integrals (src/ct_projector_2d.cu:416-440)."""
+    name = "forward_project_2d_on_the_fly_cuda"
+    _check_f32(name, img, src, dst, M_inv, b)
+    batch, (nr, nc) = _vol_dims(name, img, 2)
+    n_ray = _rays(name, src, dst, 2)
+    _affine(name, M_inv, b, 2)
+    out = torch.empty((batch, n_ray), device=img.device, dtype=torch.float32)
+    with torch.cuda.device(img.device):
+        N.call("ctlab_forward_project_2d_otf", N.ptr(img), batch, nr, nc, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), mode, zero_rule, N.ptr(out), N.stream_of(img))
+    return out
+
+
+def back_project_2d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_row, n_col, *, mode: int = N.MODE_OTF,
+                                    zero_rule: int = N.ZERO_THRESH):
+    """[B, n_row, n_col] back projection (src/ct_projector_2d.cu:442-465)."""
+    name = "back_project_2d_on_the_fly_cuda"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
+    _affine(name, M_inv, b, 2)
+    out = torch.empty((batch, int(n_row), int(n_col)), device=sino.device, dtype=torch.float32)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_2d_otf", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
+               N.ptr(b), int(n_row), int(n_col), mode, zero_rule, 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Segment lists (parity instrument, no reference counterpart)
+# ---------------------------------------------------------------------------
+
+
+def trace_segments(shape, src, dst, M_inv, b, *, mode: int = N.MODE_TORCH, zero_rule: int | None = None,
+                   cap: int | None = None):
+    """Per-ray (flat index, length) of every segment with length > 0.
+
+    Returns (idx int32 [n_ray, cap], len f32 [n_ray, cap], counts int32 [n_ray])."""
+    name = "trace_segments"
+    nd = len(shape)
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, nd)
+    _affine(name, M_inv, b, nd)
+    if zero_rule is None:
+        zero_rule = N.ZERO_EXACT if (nd == 2 and mode == N.MODE_TORCH) else N.ZERO_THRESH
+    if cap is None:
+        cap = int(sum(shape)) + nd + 2
+    dev = src.device
+    idx = torch.zeros((n_ray, cap), dtype=torch.int32, device=dev)
+    ln = torch.zeros((n_ray, cap), dtype=torch.float32, device=dev)
+    cnt = torch.zeros((n_ray,), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        if nd == 3:
+            N.call("ctlab_trace_3d", int(shape[0]), int(shape[1]), int(shape[2]), N.ptr(src), N.ptr(dst), n_ray,
+                   N.ptr(M_inv), N.ptr(b), mode, zero_rule, cap, N.ptr(idx), N.ptr(ln), N.ptr(cnt),
+                   N.stream_of(src))
+        else:
+            N.call("ctlab_trace_2d", int(shape[0]), int(shape[1]), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
+                   N.ptr(b), mode, zero_rule, cap, N.ptr(idx), N.ptr(ln), N.ptr(cnt), N.stream_of(src))
+    return idx, ln, cnt

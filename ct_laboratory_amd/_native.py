@@ -1,0 +1,86 @@
+"""ctypes binding of libctlab_hip.so (the C ABI declared in include/ctlab_hip.h).
+
+The library is built in-tree by ``python -m ct_laboratory_amd.build`` into
+``ct_laboratory_amd/_lib/libctlab_hip.so``.  There is no fallback: if the
+library is missing or fails to load, every projector call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+from .build import lib_path
+
+_lock = threading.Lock()
+_lib = None
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_INT = ctypes.c_int
+
+MODE_TORCH = 0
+MODE_OTF = 1
+ZERO_THRESH = 0
+ZERO_EXACT = 1
+
+MODES = {"torch": MODE_TORCH, "otf": MODE_OTF}
+
+# name -> argtypes (restype is int status unless listed in _RESTYPES)
+_SIGS = {
+    "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _P, _P],
+    "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _INT, _P, _P],
+    "ctlab_forward_project_2d_otf": [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _P, _P],
+    "ctlab_back_project_2d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _INT, _INT, _INT, _P, _P],
+    "ctlab_trace_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
+    "ctlab_trace_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
+    "ctlab_version": [],
+    "ctlab_last_error": [],
+}
+_RESTYPES = {"ctlab_last_error": ctypes.c_char_p}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def load() -> ctypes.CDLL:
+    """Loads the library (once).  Raises RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"libctlab_hip.so not found at {path}; build it with `python -m ct_laboratory_amd.build`"
+            )
+        L = ctypes.CDLL(path)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def call(name: str, *args) -> None:
+    """Calls an entry point; raises RuntimeError with the library's message on failure."""
+    L = load()
+    st = getattr(L, name)(*args)
+    if st != 0:
+        msg = L.ctlab_last_error()
+        raise RuntimeError(f"{name} failed (status {st}): {msg.decode() if msg else ''}")
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(t: torch.Tensor):
+    """The current torch stream on t's device, as the library's stream handle."""
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)

@@ -1,0 +1,46 @@
+// ctlab_common.h -- shared host-side helpers of libctlab_hip.so: status codes,
+// the thread-local error message, argument checks and launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ctlab_hip.h"
+
+namespace ctlab {
+
+// Set the thread-local message and return the status (used as `return fail(...)`).
+int fail(int status, const char* fmt, ...);
+const char* last_error();
+
+inline hipStream_t as_stream(ctlab_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Launch-error check after a kernel launch.
+inline int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CTLAB_EHIP, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return CTLAB_OK;
+}
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+// Host-side limits: flat voxel indices are 32-bit, ray indices 64-bit.
+constexpr int64_t kMaxVoxels = 0x7fffffffLL;
+constexpr int64_t kMaxGridX = 0x7fffffffLL;
+
+// Fills n floats with value on the stream (a kernel, never hipMemsetAsync, so
+// that the call is safe inside a captured graph).
+int fill_f32(float* p, int64_t n, float value, hipStream_t stream);
+
+}  // namespace ctlab
+
+#define CTLAB_REQUIRE(cond, ...)                                   \
+    do {                                                           \
+        if (!(cond)) return ::ctlab::fail(CTLAB_EINVAL, __VA_ARGS__); \
+    } while (0)

@@ -1,0 +1,251 @@
+// ctlab_otf.hip -- on-the-fly Siddon forward/back projection (2D and 3D) and
+// the per-ray segment-list instrument.
+//
+// One lane traces one ray (siddon_traverse.h).  The forward kernel gathers
+// volume values along the ray and writes one line integral per lane; the back
+// kernel scatters sino*length with float atomic adds (global_atomic_add_f32,
+// no return value).  blockIdx.y is the batch index, so the traversal of a ray
+// is repeated per batch member but each lane's gathers stay in one volume.
+#include "ctlab_common.h"
+#include "siddon_traverse.h"
+
+namespace ctlab {
+
+template <int ND>
+struct Dims {
+    int n[ND];
+};
+
+template <int ND>
+__device__ __forceinline__ void load_ray(const float* __restrict__ src, const float* __restrict__ dst, int64_t r,
+                                         float* s, float* d) {
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        s[a] = src[r * ND + a];
+        d[a] = dst[r * ND + a];
+    }
+}
+
+template <int ND, int MODE>
+__global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict__ vol, float* __restrict__ out,
+                                                        int64_t n_ray, int64_t vol_elems, Dims<ND> dims,
+                                                        const float* __restrict__ src,
+                                                        const float* __restrict__ dst,
+                                                        const float* __restrict__ minv,
+                                                        const float* __restrict__ b, int zero_rule) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_ray) return;
+    const int64_t bi = blockIdx.y;
+    const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
+    float s[ND], d[ND];
+    load_ray<ND>(src, dst, r, s, d);
+    const float* __restrict__ v = vol + bi * vol_elems;
+    float acc = 0.0f;
+    auto gather = [&](int idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
+    if (g.diag)
+        trace_ray<ND, MODE, true>(g, s, d, gather);
+    else
+        trace_ray<ND, MODE, false>(g, s, d, gather);
+    out[bi * n_ray + r] = acc;
+}
+
+template <int ND, int MODE>
+__global__ __launch_bounds__(kBlock) void k_back_otf(const float* __restrict__ sino, float* __restrict__ out,
+                                                     int64_t n_ray, int64_t vol_elems, Dims<ND> dims,
+                                                     const float* __restrict__ src, const float* __restrict__ dst,
+                                                     const float* __restrict__ minv, const float* __restrict__ b,
+                                                     int zero_rule) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_ray) return;
+    const int64_t bi = blockIdx.y;
+    const float val = sino[bi * n_ray + r];
+    if (val == 0.0f) return;  // every contribution would be exactly zero
+    const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
+    float s[ND], d[ND];
+    load_ray<ND>(src, dst, r, s, d);
+    float* __restrict__ o = out + bi * vol_elems;
+    auto scatter = [&](int idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
+    if (g.diag)
+        trace_ray<ND, MODE, true>(g, s, d, scatter);
+    else
+        trace_ray<ND, MODE, false>(g, s, d, scatter);
+}
+
+template <int ND, int MODE>
+__global__ __launch_bounds__(kBlock) void k_trace(int64_t n_ray, Dims<ND> dims, const float* __restrict__ src,
+                                                  const float* __restrict__ dst, const float* __restrict__ minv,
+                                                  const float* __restrict__ b, int zero_rule, int64_t cap,
+                                                  int32_t* __restrict__ idx_out, float* __restrict__ len_out,
+                                                  int32_t* __restrict__ counts) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_ray) return;
+    const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
+    float s[ND], d[ND];
+    load_ray<ND>(src, dst, r, s, d);
+    int32_t* io = idx_out + r * cap;
+    float* lo = len_out + r * cap;
+    int64_t k = 0;
+    auto record = [&](int idx, float len) {
+        if (k < cap) {
+            io[k] = idx;
+            lo[k] = len;
+        }
+        ++k;
+    };
+    if (g.diag)
+        trace_ray<ND, MODE, true>(g, s, d, record);
+    else
+        trace_ray<ND, MODE, false>(g, s, d, record);
+    counts[r] = (int32_t)k;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+template <int ND>
+static int check_common(const char* fn, const int64_t* n, int64_t batch, int64_t n_ray, const void* src,
+                        const void* dst, const void* minv, const void* b, int mode) {
+    int64_t vox = 1;
+    for (int a = 0; a < ND; ++a) {
+        CTLAB_REQUIRE(n[a] >= 1 && n[a] <= 0x7fffffffLL, "%s: grid dimension %d is %lld (must be >= 1)", fn, a,
+                      (long long)n[a]);
+        vox *= n[a];
+        CTLAB_REQUIRE(vox <= kMaxVoxels, "%s: grid has more than 2^31-1 voxels", fn);
+    }
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range [1, 65535]", fn, (long long)batch);
+    CTLAB_REQUIRE(n_ray >= 0, "%s: n_ray %lld < 0", fn, (long long)n_ray);
+    CTLAB_REQUIRE(grid_for(n_ray) <= kMaxGridX, "%s: too many rays", fn);
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    if (n_ray > 0) {
+        CTLAB_REQUIRE(src && dst, "%s: src/dst is null", fn);
+        CTLAB_REQUIRE(minv && b, "%s: M_inv/b is null", fn);
+    }
+    return CTLAB_OK;
+}
+
+template <int ND>
+static Dims<ND> make_dims(const int64_t* n) {
+    Dims<ND> d;
+    for (int a = 0; a < ND; ++a) d.n[a] = (int)n[a];
+    return d;
+}
+
+template <int ND>
+static int forward_otf(const char* fn, const float* vol, int64_t batch, const int64_t* n, const float* src,
+                       const float* dst, int64_t n_ray, const float* minv, const float* b, int mode,
+                       int zero_rule, float* out, hipStream_t stream) {
+    int st = check_common<ND>(fn, n, batch, n_ray, src, dst, minv, b, mode);
+    if (st) return st;
+    if (n_ray == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(vol && out, "%s: volume/out is null", fn);
+    int64_t V = 1;
+    for (int a = 0; a < ND; ++a) V *= n[a];
+    dim3 grid(grid_for(n_ray), (unsigned)batch);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, vol, out, n_ray, V,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+    else
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, vol, out, n_ray, V,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+    return check_launch(fn);
+}
+
+template <int ND>
+static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                    const float* dst, const float* minv, const float* b, const int64_t* n, int mode,
+                    int zero_rule, int accumulate, float* out, hipStream_t stream) {
+    int st = check_common<ND>(fn, n, batch, n_ray, src, dst, minv, b, mode);
+    if (st) return st;
+    CTLAB_REQUIRE(out, "%s: out is null", fn);
+    int64_t V = 1;
+    for (int a = 0; a < ND; ++a) V *= n[a];
+    if (!accumulate) {
+        st = fill_f32(out, V * batch, 0.0f, stream);
+        if (st) return st;
+    }
+    if (n_ray == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(sino, "%s: sino is null", fn);
+    dim3 grid(grid_for(n_ray), (unsigned)batch);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL((k_back_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, sino, out, n_ray, V,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+    else
+        hipLaunchKernelGGL((k_back_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, sino, out, n_ray, V,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+    return check_launch(fn);
+}
+
+template <int ND>
+static int trace(const char* fn, const int64_t* n, const float* src, const float* dst, int64_t n_ray,
+                 const float* minv, const float* b, int mode, int zero_rule, int64_t cap, int32_t* idx,
+                 float* len, int32_t* counts, hipStream_t stream) {
+    int st = check_common<ND>(fn, n, 1, n_ray, src, dst, minv, b, mode);
+    if (st) return st;
+    CTLAB_REQUIRE(cap >= 0, "%s: cap < 0", fn);
+    if (n_ray == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(counts && (cap == 0 || (idx && len)), "%s: output is null", fn);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL((k_trace<ND, MODE_TORCH>), dim3(grid_for(n_ray)), dim3(kBlock), 0, stream, n_ray,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule, cap, idx, len, counts);
+    else
+        hipLaunchKernelGGL((k_trace<ND, MODE_OTF>), dim3(grid_for(n_ray)), dim3(kBlock), 0, stream, n_ray,
+                           make_dims<ND>(n), src, dst, minv, b, zero_rule, cap, idx, len, counts);
+    return check_launch(fn);
+}
+
+}  // namespace ctlab
+
+using namespace ctlab;
+
+extern "C" int ctlab_forward_project_3d_otf(const float* volume, int64_t batch, int64_t n_x, int64_t n_y,
+                                            int64_t n_z, const float* src, const float* dst, int64_t n_ray,
+                                            const float* M_inv, const float* b, int mode, float* out,
+                                            ctlab_stream_t stream) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    return forward_otf<3>("forward_project_3d_otf", volume, batch, n, src, dst, n_ray, M_inv, b, mode,
+                          CTLAB_ZERO_THRESH, out, as_stream(stream));
+}
+
+extern "C" int ctlab_back_project_3d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                                         const float* dst, const float* M_inv, const float* b, int64_t n_x,
+                                         int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
+                                         ctlab_stream_t stream) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    return back_otf<3>("back_project_3d_otf", sino, batch, n_ray, src, dst, M_inv, b, n, mode,
+                       CTLAB_ZERO_THRESH, accumulate, out, as_stream(stream));
+}
+
+extern "C" int ctlab_forward_project_2d_otf(const float* image, int64_t batch, int64_t n_row, int64_t n_col,
+                                            const float* src, const float* dst, int64_t n_ray,
+                                            const float* M_inv, const float* b, int mode, int zero_rule,
+                                            float* out, ctlab_stream_t stream) {
+    const int64_t n[2] = {n_row, n_col};
+    return forward_otf<2>("forward_project_2d_otf", image, batch, n, src, dst, n_ray, M_inv, b, mode, zero_rule,
+                          out, as_stream(stream));
+}
+
+extern "C" int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                                         const float* dst, const float* M_inv, const float* b, int64_t n_row,
+                                         int64_t n_col, int mode, int zero_rule, int accumulate, float* out,
+                                         ctlab_stream_t stream) {
+    const int64_t n[2] = {n_row, n_col};
+    return back_otf<2>("back_project_2d_otf", sino, batch, n_ray, src, dst, M_inv, b, n, mode, zero_rule,
+                       accumulate, out, as_stream(stream));
+}
+
+extern "C" int ctlab_trace_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* src, const float* dst,
+                              int64_t n_ray, const float* M_inv, const float* b, int mode, int zero_rule,
+                              int64_t cap, int32_t* idx, float* len, int32_t* counts, ctlab_stream_t stream) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    return trace<3>("trace_3d", n, src, dst, n_ray, M_inv, b, mode, zero_rule, cap, idx, len, counts,
+                    as_stream(stream));
+}
+
+extern "C" int ctlab_trace_2d(int64_t n_row, int64_t n_col, const float* src, const float* dst, int64_t n_ray,
+                              const float* M_inv, const float* b, int mode, int zero_rule, int64_t cap,
+                              int32_t* idx, float* len, int32_t* counts, ctlab_stream_t stream) {
+    const int64_t n[2] = {n_row, n_col};
+    return trace<2>("trace_2d", n, src, dst, n_ray, M_inv, b, mode, zero_rule, cap, idx, len, counts,
+                    as_stream(stream));
+}

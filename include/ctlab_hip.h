@@ -1,0 +1,107 @@
+/*
+ * ctlab_hip.h -- C ABI of libctlab_hip.so, the MI355X (gfx950) Siddon projector.
+ *
+ * This library replaces the reference's pybind11 module ct_laboratory._C
+ * (src/bindings.cpp:65-104).  Every entry point takes plain device pointers,
+ * 64-bit sizes and a hipStream_t (passed as void*; NULL = the default stream),
+ * launches asynchronously on that stream and returns a status code.  Outputs are
+ * caller-allocated device buffers; the library allocates nothing except where a
+ * function says so.  On a nonzero status, ctlab_last_error() returns a message
+ * (thread-local, valid until the next call on the same thread).
+ *
+ * Conventions (as the reference):
+ *   volume  [B, n_x, n_y, n_z] float32 C-contiguous, flat = (i*n_y + j)*n_z + k
+ *           (src/ct_projector_3d.cu:149-152); 2D image [B, n_row, n_col]
+ *           (src/ct_projector_2d.cu:110).
+ *   rays    src/dst [n_ray, 3] (or [n_ray, 2]) float32 world coordinates,
+ *           P(t) = src + t (dst - src), t in [0, 1].
+ *   M_inv   row-major float32 inverse of the voxel->world matrix M (world = M idx + b),
+ *           b float32 [3] (or [2]); both device pointers (README.md:111-114).
+ *   sinogram [B, n_ray] float32.
+ *
+ * Semantics flag `mode` (on-the-fly entry points):
+ *   CTLAB_MODE_TORCH  segments of ct_projector_{2d,3d}_torch.py: crossing
+ *                     t = (p-0.5-s)/dir, length |P(t1)-P(t0)|, no endpoints.
+ *   CTLAB_MODE_OTF    segments of the CUDA on-the-fly kernel
+ *                     (src/ct_projector_3d.cu:834-1012): t = (p-0.5-s)*(1/dir),
+ *                     endpoints 0 and 1 inserted, length (t1-t0)|dst-src|.
+ * In both, a segment's voxel is round-half-even of M_inv (midpoint - b).
+ */
+#ifndef CTLAB_HIP_H
+#define CTLAB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTLAB_OK 0
+#define CTLAB_EINVAL 1   /* bad argument (shape, size, null pointer) */
+#define CTLAB_EHIP 2     /* HIP runtime / launch error */
+#define CTLAB_ENOTIMPL 3 /* entry point not implemented */
+
+#define CTLAB_MODE_TORCH 0
+#define CTLAB_MODE_OTF 1
+
+#define CTLAB_ZERO_THRESH 0 /* |dir| < 1e-12: axis has no crossings (3D torch, all CUDA kernels) */
+#define CTLAB_ZERO_EXACT 1  /* dir == 0: axis has no crossings (ct_projector_2d_torch.py:42) */
+
+typedef void* ctlab_stream_t; /* hipStream_t */
+
+/* Library identification; no counterpart in the reference. */
+int ctlab_version(void);
+const char* ctlab_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* 3D on-the-fly (no intersection table)                                   */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces _C.forward_project_3d_on_the_fly_cuda (bindings.cpp:98-100;
+ * src/ct_projector_3d.cu:1018-1049).  out [batch, n_ray] is overwritten. */
+int ctlab_forward_project_3d_otf(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
+                                 const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                 const float* b, int mode, float* out, ctlab_stream_t stream);
+
+/* Replaces _C.back_project_3d_on_the_fly_cuda (bindings.cpp:101-103;
+ * src/ct_projector_3d.cu:1051-1079).  out [batch, n_x, n_y, n_z]: zeroed first
+ * unless accumulate != 0, in which case the projection is added to it. */
+int ctlab_back_project_3d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                              const float* dst, const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
+                              int64_t n_z, int mode, int accumulate, float* out, ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* 2D on-the-fly                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces _C.forward_project_2d_on_the_fly_cuda (bindings.cpp:75-77;
+ * src/ct_projector_2d.cu:416-440).  zero_rule selects the no-crossing test. */
+int ctlab_forward_project_2d_otf(const float* image, int64_t batch, int64_t n_row, int64_t n_col,
+                                 const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                 const float* b, int mode, int zero_rule, float* out, ctlab_stream_t stream);
+
+/* Replaces _C.back_project_2d_on_the_fly_cuda (bindings.cpp:78-80;
+ * src/ct_projector_2d.cu:442-465). */
+int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                              const float* dst, const float* M_inv, const float* b, int64_t n_row,
+                              int64_t n_col, int mode, int zero_rule, int accumulate, float* out,
+                              ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Per-ray segment lists (parity instrument; no counterpart in the         */
+/* reference): for each ray the flat voxel index and length of every        */
+/* segment with length > 0, in traversal order.  idx/len are [n_ray, cap];  */
+/* counts [n_ray] receives the full count even when it exceeds cap.         */
+/* ---------------------------------------------------------------------- */
+int ctlab_trace_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* src, const float* dst, int64_t n_ray,
+                   const float* M_inv, const float* b, int mode, int zero_rule, int64_t cap, int32_t* idx,
+                   float* len, int32_t* counts, ctlab_stream_t stream);
+int ctlab_trace_2d(int64_t n_row, int64_t n_col, const float* src, const float* dst, int64_t n_ray,
+                   const float* M_inv, const float* b, int mode, int zero_rule, int64_t cap, int32_t* idx,
+                   float* len, int32_t* counts, ctlab_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTLAB_HIP_H */

@@ -1,0 +1,167 @@
+"""GPU parity of the on-the-fly projectors (2D and 3D) against the oracle.
+
+Bar: per-ray segment lists (voxel index and length of every segment with
+length > 0) bit-exact; line integrals and back projections within 1e-5
+relative (floor 1e-3 of the max); adjoint |<Ax,y>-<x,A^T y>|/|<Ax,y>| < 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from ct_laboratory_amd import _C
+from ct_laboratory_amd import _native as N
+from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+from helpers import lattice_rays, positive_lists, random_geometry, random_rays, rel_err
+
+pytestmark = pytest.mark.gpu
+
+MODES = [("torch", N.MODE_TORCH), ("otf", N.MODE_OTF)]
+
+
+def _t(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
+
+
+def _gpu_lists(shape, minv, b, src, dst, mode, dev, zero_rule=None):
+    idx, ln, cnt = _C.trace_segments(shape, _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev), mode=mode,
+                                     zero_rule=zero_rule)
+    return positive_lists(idx.cpu().numpy(), ln.cpu().numpy(), cnt.cpu().numpy())
+
+
+def _assert_lists_equal(shape, minv, b, src, dst, mname, mode, dev, zero_rule=None):
+    nd = len(shape)
+    zr = zero_rule if zero_rule is not None else (N.ZERO_EXACT if (nd == 2 and mname == "torch") else N.ZERO_THRESH)
+    g = _gpu_lists(shape, minv, b, src, dst, mode, dev, zero_rule=zr)
+    oi, ol, oc = O.trace(shape, minv, b, src, dst, mode=mname, zero_rule="exact" if zr == N.ZERO_EXACT else "thresh")
+    o = positive_lists(oi, ol, oc)
+    bad = []
+    for r in range(len(o)):
+        gi, gl = g[r]
+        ri, rl = o[r]
+        if len(gi) != len(ri) or (gi != ri).any() or (gl.view(np.int32) != rl.view(np.int32)).any():
+            bad.append(r)
+    assert not bad, f"{len(bad)} rays differ, first {bad[:5]}: gpu {g[bad[0]][0][:8]} oracle {o[bad[0]][0][:8]}"
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("mname,mode", MODES)
+@pytest.mark.parametrize("kind", ["far", "inside", "lattice"])
+def test_segment_lists_bit_exact_random(dev, nd, mname, mode, kind):
+    rng = np.random.default_rng(1000 + nd * 10 + mode)
+    for trial in range(4):
+        shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial == 3))
+        src, dst = random_rays(rng, shape, s, 3000, kind)
+        _assert_lists_equal(shape, minv, b, src, dst, mname, mode, dev)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_segment_lists_bit_exact_degenerate(dev, nd, mname, mode):
+    """Rays on voxel planes, through corners and axis-parallel (H6/H7 ties)."""
+    rng = np.random.default_rng(7 + nd)
+    shape = (9, 9, 9) if nd == 3 else (13, 11)
+    src, dst = lattice_rays(rng, nd, 20000)
+    eye = np.eye(nd, dtype=np.float32)
+    _assert_lists_equal(shape, eye, np.zeros(nd, np.float32), src, dst, mname, mode, dev)
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_segment_lists_conebeam(dev, mname, mode):
+    n = 64
+    src, dst = build_conebeam_3d_geometry(24, 48, 40, 1000.0, 1500.0, det_spacing=6.0)
+    b = np.full(3, -(n - 1) / 2 * 1.0, np.float32)
+    _assert_lists_equal((n, n, n), np.eye(3, dtype=np.float32), b, src.numpy(), dst.numpy(), mname, mode, dev)
+
+
+def test_closed_forms(dev):
+    """Known answers from the reference's python_examples (as asserts)."""
+    img = np.zeros((32, 32), np.float32)
+    img[16, :] = 1
+    e2, z2 = _t(np.eye(2), dev), _t(np.zeros(2), dev)
+    for mode in (N.MODE_TORCH, N.MODE_OTF):
+        out = _C.forward_project_2d_on_the_fly_cuda(_t(img, dev), _t([[16.0, -5.0]], dev), _t([[16.0, 37.0]], dev),
+                                                    e2, z2, mode=mode).cpu().numpy()
+        assert abs(out[0, 0] - 32.0) < 1e-4  # test_ct_projector_2d_line_integral.py:31-43,70
+    img = np.zeros((16, 16), np.float32)
+    img[8, :] = 1
+    src = [[8.0, -5.0], [0, 0], [15, 15]]
+    dst = [[8.0, 25.0], [15, 15], [0, 8]]
+    out = _C.forward_project_2d_on_the_fly_cuda(_t(img, dev), _t(src, dev), _t(dst, dev), e2, z2,
+                                                mode=N.MODE_TORCH, zero_rule=N.ZERO_EXACT).cpu().numpy()[0]
+    np.testing.assert_allclose(out, [16.0, math.sqrt(2), math.sqrt(15**2 + 7**2) / 15], rtol=1e-6)
+    vol = np.zeros((32, 32, 32), np.float32)
+    vol[:, 16, 16] = 1
+    e3, z3 = _t(np.eye(3), dev), _t(np.zeros(3), dev)
+    out = _C.forward_project_3d_on_the_fly_cuda(_t(vol, dev), _t([[200.0, 16, 16]], dev), _t([[-200.0, 16, 16]], dev),
+                                                e3, z3, mode=N.MODE_TORCH).cpu().numpy()
+    assert abs(out[0, 0] - 32.0) < 1e-4  # test_ct_projector_3d_line_integral.py:33-45,74
+    vol = np.zeros((8, 8, 8), np.float32)
+    vol[4] = 1
+    src = [[4.0, -5, 4], [0, 0, 0]]
+    dst = [[4.0, 15, 4], [7, 7, 7]]
+    for mode in (N.MODE_TORCH, N.MODE_OTF):
+        out = _C.forward_project_3d_on_the_fly_cuda(_t(vol, dev), _t(src, dev), _t(dst, dev), e3, z3,
+                                                    mode=mode).cpu().numpy()[0]
+        np.testing.assert_allclose(out, [8.0, math.sqrt(3)], rtol=1e-6)  # test_ct_projector_3d_module.py:17-32
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_forward_back_vs_oracle(dev, nd, mname, mode):
+    rng = np.random.default_rng(50 + nd + 3 * mode)
+    shape, M, minv, b, s = random_geometry(rng, nd, max_n=40)
+    src, dst = random_rays(rng, shape, s, 5000, "far")
+    zr = N.ZERO_EXACT if (nd == 2 and mname == "torch") else N.ZERO_THRESH
+    ozr = "exact" if zr == N.ZERO_EXACT else "thresh"
+    B = 2
+    vol = rng.normal(size=(B,) + shape).astype(np.float32)
+    sino = rng.normal(size=(B, src.shape[0])).astype(np.float32)
+    tv, ts, td, tm, tb = _t(vol, dev), _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev)
+    if nd == 3:
+        fwd = _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb, mode=mode)
+        bk = _C.back_project_3d_on_the_fly_cuda(_t(sino, dev), ts, td, tm, tb, *shape, mode=mode)
+    else:
+        fwd = _C.forward_project_2d_on_the_fly_cuda(tv, ts, td, tm, tb, mode=mode, zero_rule=zr)
+        bk = _C.back_project_2d_on_the_fly_cuda(_t(sino, dev), ts, td, tm, tb, *shape, mode=mode, zero_rule=zr)
+    ref_f = O.forward(vol, minv, b, src, dst, mode=mname, zero_rule=ozr)
+    ref_b = O.back(sino, shape, minv, b, src, dst, mode=mname, zero_rule=ozr)
+    assert rel_err(fwd.cpu().numpy(), ref_f) < 1e-5
+    assert rel_err(bk.cpu().numpy(), ref_b) < 1e-5
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_adjoint_conebeam(dev, mname, mode):
+    n = 48
+    src, dst = build_conebeam_3d_geometry(30, 40, 36, 1000.0, 1500.0, det_spacing=2.5)
+    b = np.full(3, -(n - 1) / 2, np.float32)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(n, n, n)).astype(np.float32)
+    y = rng.normal(size=src.shape[0]).astype(np.float32)
+    tm, tb = _t(np.eye(3), dev), _t(b, dev)
+    ts, td = src.to(dev), dst.to(dev)
+    Ax = _C.forward_project_3d_on_the_fly_cuda(_t(x, dev), ts, td, tm, tb, mode=mode)[0].cpu().numpy()
+    ATy = _C.back_project_3d_on_the_fly_cuda(_t(y, dev), ts, td, tm, tb, n, n, n, mode=mode)[0].cpu().numpy()
+    lhs = float(np.dot(Ax.astype(np.float64), y.astype(np.float64)))
+    rhs = float(np.dot(x.astype(np.float64).ravel(), ATy.astype(np.float64).ravel()))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+    if mname == "torch":
+        ref = O.forward_f64(x.astype(np.float64), np.eye(3), b.astype(np.float64), src.numpy().astype(np.float64),
+                            dst.numpy().astype(np.float64))
+        lhs64 = float(np.dot(ref, y.astype(np.float64)))
+        assert abs(lhs - lhs64) / abs(lhs64) < 1e-5
+
+
+def test_empty_and_zero(dev):
+    z = torch.zeros((0, 3), device=dev)
+    vol = torch.ones((4, 4, 4), device=dev)
+    e3, z3 = torch.eye(3, device=dev), torch.zeros(3, device=dev)
+    out = _C.forward_project_3d_on_the_fly_cuda(vol, z, z, e3, z3)
+    assert out.shape == (1, 0)
+    bk = _C.back_project_3d_on_the_fly_cuda(torch.zeros((0,), device=dev), z, z, e3, z3, 4, 4, 4)
+    assert bk.shape == (1, 4, 4, 4) and float(bk.abs().sum()) == 0.0
+    # zero-length ray
+    p = torch.tensor([[1.0, 1.0, 1.0]], device=dev)
+    assert float(_C.forward_project_3d_on_the_fly_cuda(vol, p, p, e3, z3)[0, 0]) == 0.0

@@ -1,0 +1,79 @@
+"""The product traversal (siddon_traverse.h), compiled for the host, against the
+oracle's sort-based restatement: segment lists bit-exact (index and length)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import lattice_rays, positive_lists, random_geometry, random_rays
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "native", "_build", "libtrace_host.so")
+
+
+@pytest.fixture(scope="module")
+def host():
+    assert os.path.exists(LIB), "run __graft_entry__.build() first"
+    L = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    L.host_trace.restype = ctypes.c_int64
+    L.host_trace.argtypes = [ctypes.c_int] * 3 + [P] * 5 + [ctypes.c_int64] * 2 + [P] * 3
+
+    def run(shape, minv, b, src, dst, mode, zr):
+        nd = len(shape)
+        n = np.array(shape, np.int32)
+        mi = np.ascontiguousarray(minv, np.float32).reshape(-1)
+        bb = np.ascontiguousarray(b, np.float32)
+        src = np.ascontiguousarray(src, np.float32)
+        dst = np.ascontiguousarray(dst, np.float32)
+        N = src.shape[0]
+        cap = sum(shape) + nd + 2
+        idx = np.zeros((N, cap), np.int64)
+        ln = np.zeros((N, cap), np.float32)
+        cnt = np.zeros(N, np.int64)
+        p = lambda a: a.ctypes.data_as(P)  # noqa: E731
+        L.host_trace(nd, {"torch": 0, "otf": 1}[mode], {"thresh": 0, "exact": 1}[zr], p(n), p(mi), p(bb), p(src),
+                     p(dst), N, cap, p(idx), p(ln), p(cnt))
+        return [(idx[r, :cnt[r]], ln[r, :cnt[r]]) for r in range(N)]
+
+    return run
+
+
+def _check(host, shape, minv, b, src, dst, mode):
+    zr = "exact" if (len(shape) == 2 and mode == "torch") else "thresh"
+    got = host(shape, minv, b, src, dst, mode, zr)
+    ref = positive_lists(*O.trace(shape, minv, b, src, dst, mode=mode, zero_rule=zr))
+    bad = [r for r in range(len(ref)) if len(got[r][0]) != len(ref[r][0]) or (got[r][0] != ref[r][0]).any()
+           or (got[r][1].view(np.int32) != ref[r][1].view(np.int32)).any()]
+    assert not bad, f"{len(bad)} rays differ (first {bad[:5]})"
+
+
+@pytest.mark.parametrize("mode", ["torch", "otf"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_random_geometries(host, mode, nd):
+    rng = np.random.default_rng(11 * nd + (mode == "otf"))
+    for trial in range(12):
+        shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial % 4 == 3))
+        kind = ["far", "inside", "lattice"][trial % 3]
+        src, dst = random_rays(rng, shape, s, 1500, kind)
+        _check(host, shape, minv, b, src, dst, mode)
+
+
+@pytest.mark.parametrize("mode", ["torch", "otf"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_plane_and_corner_rays(host, mode, nd):
+    rng = np.random.default_rng(3 + nd)
+    shape = (9, 9, 9) if nd == 3 else (13, 11)
+    src, dst = lattice_rays(rng, nd, 20000)
+    _check(host, shape, np.eye(nd), np.zeros(nd), src, dst, mode)
+
+
+@pytest.mark.parametrize("mode", ["torch", "otf"])
+def test_conebeam_rays(host, mode):
+    from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+
+    src, dst = build_conebeam_3d_geometry(10, 40, 24, 1000.0, 1500.0, det_spacing=1.5 * 256 / 40)
+    b = np.full(3, -(256 - 1) / 2, np.float32)
+    _check(host, (256, 256, 256), np.eye(3), b, src.numpy(), dst.numpy(), mode)

@@ -1,0 +1,44 @@
+"""CPU-side checks of the C ABI: the library loads and exports every entry point
+declared in include/ctlab_hip.h, and the ctypes binding declares the same set."""
+import ctypes
+import os
+import re
+
+from ct_laboratory_amd import _native
+from ct_laboratory_amd.build import lib_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "ctlab_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ctlab_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(lib_path()), "build the library first (python -m ct_laboratory_amd.build)"
+    lib = ctypes.CDLL(lib_path())
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_symbols()) == set(_native.exported_symbols())
+
+
+def test_version_and_error_string():
+    L = _native.load()
+    assert L.ctlab_version() >= 1
+    assert isinstance(L.ctlab_last_error(), bytes)
+
+
+def test_argument_errors_are_reported_without_gpu():
+    """Argument checks run on the host before any HIP call."""
+    L = _native.load()
+    st = L.ctlab_forward_project_3d_otf(None, 1, 0, 4, 4, None, None, 1, None, None, 0, None, None)
+    assert st == 1 and b"grid dimension" in L.ctlab_last_error()
+    st = L.ctlab_back_project_3d_otf(None, 1, 1, None, None, None, None, 4, 4, 4, 7, 0, None, None)
+    assert st == 1 and b"mode" in L.ctlab_last_error()
+    st = L.ctlab_forward_project_2d_otf(None, 0, 4, 4, None, None, 1, None, None, 0, 0, None, None)
+    assert st == 1 and b"batch" in L.ctlab_last_error()
