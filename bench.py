@@ -100,6 +100,8 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["torch", "otf"], default="torch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--plain-order", action="store_true", help="one ray per lane in storage order (no tiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the full-size adjoint check")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projector shards over rays "
@@ -124,13 +126,15 @@ def main(argv=None) -> int:
     del cnt
     print(f"[bench] n_ray={n_ray} segments={S}", file=sys.stderr, flush=True)
 
+    det = None if a.plain_order else (CFG["det_ny"], CFG["det_nx"])
+
     def step(ev=None):
         if ev is not None:
             ev[0].record()
-        sino = _C.forward_project_3d_on_the_fly_cuda(vol, src, dst, minv, b, mode=mode)
+        sino = _C.forward_project_3d_on_the_fly_cuda(vol, src, dst, minv, b, mode=mode, det_shape=det)
         if ev is not None:
             ev[1].record()
-        bk = _C.back_project_3d_on_the_fly_cuda(sino, src, dst, minv, b, n, n, n, mode=mode)
+        bk = _C.back_project_3d_on_the_fly_cuda(sino, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
         if ev is not None:
             ev[2].record()
         return sino, bk
@@ -150,6 +154,16 @@ def main(argv=None) -> int:
     total_s = t1 - t0
     ms_per_step = 1e3 * total_s / a.steps
     value = n_ray * a.steps / total_s
+
+    adjoint = None
+    if not a.no_adjoint:
+        # Config check: <A x, y> vs <x, A^T y> with y ~ N(0, 1) (seed 2), FP64 dot products.
+        y = torch.from_numpy(np.random.default_rng(2).standard_normal(n_ray, dtype=np.float32)).to(dev)
+        aty = _C.back_project_3d_on_the_fly_cuda(y, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
+        lhs = float(torch.dot(sino[0].double(), y.double()))
+        rhs = float(torch.dot(vol.reshape(-1).double(), aty.reshape(-1).double()))
+        adjoint = abs(lhs - rhs) / abs(lhs)
+        del y, aty
 
     fwd_bytes = 4 * S + 4 * n_ray + 24 * n_ray
     back_bytes = 4 * S + 4 * n_ray + 24 * n_ray
@@ -176,6 +190,8 @@ def main(argv=None) -> int:
             "n_ray": n_ray,
             "segments": S,
             "semantics": a.mode,
+            "schedule": "plain" if a.plain_order else "detector-tiled (64 rows x 4 cols per block)",
+            "adjoint_rel_err": adjoint,
             "fwd_ms": fwd_ms,
             "back_ms": back_ms,
             "voxel_updates_per_s": 2 * S * a.steps / total_s,

@@ -68,8 +68,11 @@ def _sino_dims(name: str, sino: torch.Tensor):
 # ---------------------------------------------------------------------------
 
 
-def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF):
-    """[B, n_ray] line integrals (src/ct_projector_3d.cu:1018-1049)."""
+def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF, det_shape=None):
+    """[B, n_ray] line integrals (src/ct_projector_3d.cu:1018-1049).
+
+    ``det_shape=(rows, cols)`` declares rays ordered [view][rows][cols] (a
+    scheduling hint; results do not depend on it)."""
     name = "forward_project_3d_on_the_fly_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
@@ -77,12 +80,14 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
     _affine(name, M_inv, b, 3)
     out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
     with torch.cuda.device(volume.device):
+        rows, cols = det_shape if det_shape is not None else (0, 0)
         N.call("ctlab_forward_project_3d_otf", N.ptr(volume), batch, nx, ny, nz, N.ptr(src), N.ptr(dst), n_ray,
-               N.ptr(M_inv), N.ptr(b), mode, N.ptr(out), N.stream_of(volume))
+               N.ptr(M_inv), N.ptr(b), mode, int(rows), int(cols), N.ptr(out), N.stream_of(volume))
     return out
 
 
-def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF):
+def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF,
+                                    det_shape=None):
     """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079)."""
     name = "back_project_3d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
@@ -91,8 +96,9 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
     _affine(name, M_inv, b, 3)
     out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
     with torch.cuda.device(sino.device):
+        rows, cols = det_shape if det_shape is not None else (0, 0)
         N.call("ctlab_back_project_3d_otf", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
-               N.ptr(b), int(n_x), int(n_y), int(n_z), mode, 0, N.ptr(out), N.stream_of(sino))
+               N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(rows), int(cols), 0, N.ptr(out), N.stream_of(sino))
     return out
 
 

@@ -30,8 +30,9 @@ MODES = {"torch": MODE_TORCH, "otf": MODE_OTF}
 
 # name -> argtypes (restype is int status unless listed in _RESTYPES)
 _SIGS = {
-    "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _P, _P],
-    "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _INT, _P, _P],
+    "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _I64, _P, _P],
+    "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I64, _I64, _INT, _P,
+                                  _P],
     "ctlab_forward_project_2d_otf": [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _P, _P],
     "ctlab_back_project_2d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _INT, _INT, _INT, _P, _P],
     "ctlab_trace_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],

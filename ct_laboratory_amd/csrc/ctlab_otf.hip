@@ -16,6 +16,44 @@ struct Dims {
     int n[ND];
 };
 
+// Lane -> ray schedule.  With a detector layout (rays ordered
+// [view][row][col], col fastest), a block is a tile of 64 rows x 4 columns of
+// one view and each wave takes 64 consecutive rows at one column.  Such rays
+// share their in-plane path and differ in the slow z direction, which is the
+// fastest-varying volume axis, so one wave's gathers or atomics touch two or
+// three cache lines instead of 64.  Blocks are remapped so that each XCD (the
+// hardware deals blocks round-robin to the 8 XCDs) receives a contiguous run
+// of tiles and neighbouring tiles share that XCD's L2.
+struct RaySched {
+    int64_t n_ray;
+    uint32_t rows, cols;        // 0: plain order, block b covers rays [256 b, 256 b + 256)
+    uint32_t tiles_r, tiles_c;  // ceil(rows / 64), ceil(cols / 4)
+    uint32_t n_blocks;          // gridDim.x
+};
+
+constexpr int kTileRows = 64;
+constexpr int kTileCols = kBlock / kTileRows;
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = bid % kXcds, j = bid / kXcds;
+    return (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
+__device__ __forceinline__ int64_t ray_of_lane(const RaySched& s) {
+    const uint32_t t = xcd_remap(blockIdx.x, s.n_blocks);
+    if (s.rows == 0) {
+        const int64_t r = (int64_t)t * kBlock + threadIdx.x;
+        return r < s.n_ray ? r : -1;
+    }
+    const uint32_t per_view = s.tiles_r * s.tiles_c;
+    const uint32_t view = t / per_view, rem = t % per_view;
+    const uint32_t row = (rem / s.tiles_c) * kTileRows + (threadIdx.x % kTileRows);
+    const uint32_t col = (rem % s.tiles_c) * kTileCols + (threadIdx.x / kTileRows);
+    if (row >= s.rows || col >= s.cols) return -1;
+    return ((int64_t)view * s.rows + row) * s.cols + col;
+}
+
 template <int ND>
 __device__ __forceinline__ void load_ray(const float* __restrict__ src, const float* __restrict__ dst, int64_t r,
                                          float* s, float* d) {
@@ -28,14 +66,15 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ src, const fl
 
 template <int ND, int MODE>
 __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict__ vol, float* __restrict__ out,
-                                                        int64_t n_ray, int64_t vol_elems, Dims<ND> dims,
+                                                        RaySched sched, int64_t vol_elems, Dims<ND> dims,
                                                         const float* __restrict__ src,
                                                         const float* __restrict__ dst,
                                                         const float* __restrict__ minv,
                                                         const float* __restrict__ b, int zero_rule) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_ray) return;
+    const int64_t r = ray_of_lane(sched);
+    if (r < 0) return;
     const int64_t bi = blockIdx.y;
+    const int64_t n_ray = sched.n_ray;
     const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
     float s[ND], d[ND];
     load_ray<ND>(src, dst, r, s, d);
@@ -51,14 +90,14 @@ __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict_
 
 template <int ND, int MODE>
 __global__ __launch_bounds__(kBlock) void k_back_otf(const float* __restrict__ sino, float* __restrict__ out,
-                                                     int64_t n_ray, int64_t vol_elems, Dims<ND> dims,
+                                                     RaySched sched, int64_t vol_elems, Dims<ND> dims,
                                                      const float* __restrict__ src, const float* __restrict__ dst,
                                                      const float* __restrict__ minv, const float* __restrict__ b,
                                                      int zero_rule) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_ray) return;
+    const int64_t r = ray_of_lane(sched);
+    if (r < 0) return;
     const int64_t bi = blockIdx.y;
-    const float val = sino[bi * n_ray + r];
+    const float val = sino[bi * sched.n_ray + r];
     if (val == 0.0f) return;  // every contribution would be exactly zero
     const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
     float s[ND], d[ND];
@@ -131,22 +170,47 @@ static Dims<ND> make_dims(const int64_t* n) {
     return d;
 }
 
+static int make_sched(const char* fn, int64_t n_ray, int64_t rows, int64_t cols, RaySched* s) {
+    s->n_ray = n_ray;
+    s->rows = s->cols = s->tiles_r = s->tiles_c = 0;
+    int64_t blocks;
+    if (rows <= 0 && cols <= 0) {
+        blocks = (n_ray + kBlock - 1) / kBlock;
+    } else {
+        CTLAB_REQUIRE(rows > 0 && cols > 0 && rows <= 0x7fffffffLL && cols <= 0x7fffffffLL,
+                      "%s: detector layout %lld x %lld invalid", fn, (long long)rows, (long long)cols);
+        CTLAB_REQUIRE(n_ray % (rows * cols) == 0, "%s: n_ray %lld is not a multiple of the detector size %lld x %lld",
+                      fn, (long long)n_ray, (long long)rows, (long long)cols);
+        s->rows = (uint32_t)rows;
+        s->cols = (uint32_t)cols;
+        s->tiles_r = (uint32_t)((rows + kTileRows - 1) / kTileRows);
+        s->tiles_c = (uint32_t)((cols + kTileCols - 1) / kTileCols);
+        blocks = (n_ray / (rows * cols)) * (int64_t)s->tiles_r * s->tiles_c;
+    }
+    CTLAB_REQUIRE(blocks <= 0xffffffffLL && blocks <= kMaxGridX, "%s: too many rays for one launch", fn);
+    s->n_blocks = (uint32_t)blocks;
+    return CTLAB_OK;
+}
+
 template <int ND>
 static int forward_otf(const char* fn, const float* vol, int64_t batch, const int64_t* n, const float* src,
                        const float* dst, int64_t n_ray, const float* minv, const float* b, int mode,
-                       int zero_rule, float* out, hipStream_t stream) {
+                       int zero_rule, int64_t rows, int64_t cols, float* out, hipStream_t stream) {
     int st = check_common<ND>(fn, n, batch, n_ray, src, dst, minv, b, mode);
+    if (st) return st;
+    RaySched sched;
+    st = make_sched(fn, n_ray, rows, cols, &sched);
     if (st) return st;
     if (n_ray == 0) return CTLAB_OK;
     CTLAB_REQUIRE(vol && out, "%s: volume/out is null", fn);
     int64_t V = 1;
     for (int a = 0; a < ND; ++a) V *= n[a];
-    dim3 grid(grid_for(n_ray), (unsigned)batch);
+    dim3 grid(sched.n_blocks, (unsigned)batch);
     if (mode == CTLAB_MODE_TORCH)
-        hipLaunchKernelGGL((k_forward_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, vol, out, n_ray, V,
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, vol, out, sched, V,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule);
     else
-        hipLaunchKernelGGL((k_forward_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, vol, out, n_ray, V,
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, vol, out, sched, V,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule);
     return check_launch(fn);
 }
@@ -154,8 +218,11 @@ static int forward_otf(const char* fn, const float* vol, int64_t batch, const in
 template <int ND>
 static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_ray, const float* src,
                     const float* dst, const float* minv, const float* b, const int64_t* n, int mode,
-                    int zero_rule, int accumulate, float* out, hipStream_t stream) {
+                    int zero_rule, int64_t rows, int64_t cols, int accumulate, float* out, hipStream_t stream) {
     int st = check_common<ND>(fn, n, batch, n_ray, src, dst, minv, b, mode);
+    if (st) return st;
+    RaySched sched;
+    st = make_sched(fn, n_ray, rows, cols, &sched);
     if (st) return st;
     CTLAB_REQUIRE(out, "%s: out is null", fn);
     int64_t V = 1;
@@ -166,12 +233,12 @@ static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_
     }
     if (n_ray == 0) return CTLAB_OK;
     CTLAB_REQUIRE(sino, "%s: sino is null", fn);
-    dim3 grid(grid_for(n_ray), (unsigned)batch);
+    dim3 grid(sched.n_blocks, (unsigned)batch);
     if (mode == CTLAB_MODE_TORCH)
-        hipLaunchKernelGGL((k_back_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, sino, out, n_ray, V,
+        hipLaunchKernelGGL((k_back_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, sino, out, sched, V,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule);
     else
-        hipLaunchKernelGGL((k_back_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, sino, out, n_ray, V,
+        hipLaunchKernelGGL((k_back_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, sino, out, sched, V,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule);
     return check_launch(fn);
 }
@@ -200,20 +267,20 @@ using namespace ctlab;
 
 extern "C" int ctlab_forward_project_3d_otf(const float* volume, int64_t batch, int64_t n_x, int64_t n_y,
                                             int64_t n_z, const float* src, const float* dst, int64_t n_ray,
-                                            const float* M_inv, const float* b, int mode, float* out,
-                                            ctlab_stream_t stream) {
+                                            const float* M_inv, const float* b, int mode, int64_t det_rows,
+                                            int64_t det_cols, float* out, ctlab_stream_t stream) {
     const int64_t n[3] = {n_x, n_y, n_z};
     return forward_otf<3>("forward_project_3d_otf", volume, batch, n, src, dst, n_ray, M_inv, b, mode,
-                          CTLAB_ZERO_THRESH, out, as_stream(stream));
+                          CTLAB_ZERO_THRESH, det_rows, det_cols, out, as_stream(stream));
 }
 
 extern "C" int ctlab_back_project_3d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
                                          const float* dst, const float* M_inv, const float* b, int64_t n_x,
-                                         int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
-                                         ctlab_stream_t stream) {
+                                         int64_t n_y, int64_t n_z, int mode, int64_t det_rows, int64_t det_cols,
+                                         int accumulate, float* out, ctlab_stream_t stream) {
     const int64_t n[3] = {n_x, n_y, n_z};
     return back_otf<3>("back_project_3d_otf", sino, batch, n_ray, src, dst, M_inv, b, n, mode,
-                       CTLAB_ZERO_THRESH, accumulate, out, as_stream(stream));
+                       CTLAB_ZERO_THRESH, det_rows, det_cols, accumulate, out, as_stream(stream));
 }
 
 extern "C" int ctlab_forward_project_2d_otf(const float* image, int64_t batch, int64_t n_row, int64_t n_col,
@@ -222,7 +289,7 @@ extern "C" int ctlab_forward_project_2d_otf(const float* image, int64_t batch, i
                                             float* out, ctlab_stream_t stream) {
     const int64_t n[2] = {n_row, n_col};
     return forward_otf<2>("forward_project_2d_otf", image, batch, n, src, dst, n_ray, M_inv, b, mode, zero_rule,
-                          out, as_stream(stream));
+                          0, 0, out, as_stream(stream));
 }
 
 extern "C" int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
@@ -230,7 +297,7 @@ extern "C" int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64
                                          int64_t n_col, int mode, int zero_rule, int accumulate, float* out,
                                          ctlab_stream_t stream) {
     const int64_t n[2] = {n_row, n_col};
-    return back_otf<2>("back_project_2d_otf", sino, batch, n_ray, src, dst, M_inv, b, n, mode, zero_rule,
+    return back_otf<2>("back_project_2d_otf", sino, batch, n_ray, src, dst, M_inv, b, n, mode, zero_rule, 0, 0,
                        accumulate, out, as_stream(stream));
 }
 

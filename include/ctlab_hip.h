@@ -58,17 +58,24 @@ const char* ctlab_last_error(void);
 /* ---------------------------------------------------------------------- */
 
 /* Replaces _C.forward_project_3d_on_the_fly_cuda (bindings.cpp:98-100;
- * src/ct_projector_3d.cu:1018-1049).  out [batch, n_ray] is overwritten. */
+ * src/ct_projector_3d.cu:1018-1049).  out [batch, n_ray] is overwritten.
+ * det_rows/det_cols (0 = none) declare that the rays are ordered
+ * [view][det_rows][det_cols] with columns fastest, as the cone-beam builder
+ * orders them (conebeam_projector_3d.py:84-96); the kernel then schedules
+ * each wavefront over 64 consecutive detector rows of one column.  Results do
+ * not depend on it. */
 int ctlab_forward_project_3d_otf(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
                                  const float* src, const float* dst, int64_t n_ray, const float* M_inv,
-                                 const float* b, int mode, float* out, ctlab_stream_t stream);
+                                 const float* b, int mode, int64_t det_rows, int64_t det_cols, float* out,
+                                 ctlab_stream_t stream);
 
 /* Replaces _C.back_project_3d_on_the_fly_cuda (bindings.cpp:101-103;
  * src/ct_projector_3d.cu:1051-1079).  out [batch, n_x, n_y, n_z]: zeroed first
  * unless accumulate != 0, in which case the projection is added to it. */
 int ctlab_back_project_3d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
                               const float* dst, const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
-                              int64_t n_z, int mode, int accumulate, float* out, ctlab_stream_t stream);
+                              int64_t n_z, int mode, int64_t det_rows, int64_t det_cols, int accumulate,
+                              float* out, ctlab_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* 2D on-the-fly                                                           */

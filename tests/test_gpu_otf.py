@@ -170,6 +170,27 @@ def test_adjoint_conebeam(dev, mname, mode):
     assert abs(lhs - lhs_o) / abs(lhs_o) < 1e-5
 
 
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_detector_tiled_schedule(dev, mname, mode):
+    """The detector-tiled lane schedule changes only which lane traces a ray."""
+    n = 40
+    src, dst = build_conebeam_3d_geometry(7, 37, 70, 1000.0, 1500.0, det_spacing=3.0)  # ragged tiles
+    b = _t(np.full(3, -(n - 1) / 2, np.float32), dev)
+    e3 = _t(np.eye(3), dev)
+    rng = np.random.default_rng(5)
+    vol = _t(rng.uniform(0, 1, (2, n, n, n)), dev)
+    ts, td = src.to(dev), dst.to(dev)
+    f0 = _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, e3, b, mode=mode)
+    f1 = _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, e3, b, mode=mode, det_shape=(70, 37))
+    assert torch.equal(f0, f1)
+    y = _t(rng.uniform(0, 1, (2, src.shape[0])), dev)
+    b0 = _C.back_project_3d_on_the_fly_cuda(y, ts, td, e3, b, n, n, n, mode=mode).cpu().numpy()
+    b1 = _C.back_project_3d_on_the_fly_cuda(y, ts, td, e3, b, n, n, n, mode=mode, det_shape=(70, 37)).cpu().numpy()
+    assert rel_err(b1, b0) < 1e-5
+    with pytest.raises(RuntimeError, match="multiple of the detector size"):
+        _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, e3, b, det_shape=(71, 37))
+
+
 def test_empty_and_zero(dev):
     z = torch.zeros((0, 3), device=dev)
     vol = torch.ones((4, 4, 4), device=dev)

@@ -36,9 +36,9 @@ def test_version_and_error_string():
 def test_argument_errors_are_reported_without_gpu():
     """Argument checks run on the host before any HIP call."""
     L = _native.load()
-    st = L.ctlab_forward_project_3d_otf(None, 1, 0, 4, 4, None, None, 1, None, None, 0, None, None)
+    st = L.ctlab_forward_project_3d_otf(None, 1, 0, 4, 4, None, None, 1, None, None, 0, 0, 0, None, None)
     assert st == 1 and b"grid dimension" in L.ctlab_last_error()
-    st = L.ctlab_back_project_3d_otf(None, 1, 1, None, None, None, None, 4, 4, 4, 7, 0, None, None)
+    st = L.ctlab_back_project_3d_otf(None, 1, 1, None, None, None, None, 4, 4, 4, 7, 0, 0, 0, None, None)
     assert st == 1 and b"mode" in L.ctlab_last_error()
     st = L.ctlab_forward_project_2d_otf(None, 0, 4, 4, None, None, 1, None, None, 0, 0, None, None)
     assert st == 1 and b"batch" in L.ctlab_last_error()
