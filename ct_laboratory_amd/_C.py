@@ -169,3 +169,194 @@ def trace_segments(shape, src, dst, M_inv, b, *, mode: int = N.MODE_TORCH, zero_
             N.call("ctlab_trace_2d", int(shape[0]), int(shape[1]), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
                    N.ptr(b), mode, zero_rule, cap, N.ptr(idx), N.ptr(ln), N.ptr(cnt), N.stream_of(src))
     return idx, ln, cnt
+
+
+# ---------------------------------------------------------------------------
+# Crossing tables (bindings.cpp:67-72, 83-88)
+# ---------------------------------------------------------------------------
+
+
+def compute_intersections_2d(n_row, n_col, src, dst, M_inv, b, *, zero_rule: int = N.ZERO_THRESH, K=None):
+    """[n_ray, (n_row+1)+(n_col+1)] sorted crossings, INF padded (src/ct_projector_2d.cu:158-177)."""
+    name = "compute_intersections_2d"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 2)
+    _affine(name, M_inv, b, 2)
+    K = int(n_row) + int(n_col) + 2 if K is None else int(K)
+    out = torch.empty((n_ray, K), device=src.device, dtype=torch.float32)
+    with torch.cuda.device(src.device):
+        N.call("ctlab_compute_intersections_2d", int(n_row), int(n_col), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
+               N.ptr(b), zero_rule, K, N.ptr(out), N.stream_of(src))
+    return out
+
+
+def compute_intersections_3d(n_x, n_y, n_z, src, dst, M_inv, b, *, zero_rule: int = N.ZERO_THRESH, K=None):
+    """[n_ray, (n_x+1)+(n_y+1)+(n_z+1)] sorted crossings, INF padded (src/ct_projector_3d.cu:381-401)."""
+    name = "compute_intersections_3d"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 3)
+    _affine(name, M_inv, b, 3)
+    K = int(n_x) + int(n_y) + int(n_z) + 3 if K is None else int(K)
+    out = torch.empty((n_ray, K), device=src.device, dtype=torch.float32)
+    with torch.cuda.device(src.device):
+        N.call("ctlab_compute_intersections_3d", int(n_x), int(n_y), int(n_z), N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), zero_rule, K, N.ptr(out), N.stream_of(src))
+    return out
+
+
+def count_crossings(shape, src, dst, M_inv, b, *, zero_rule: int = N.ZERO_THRESH):
+    """int32 [n_ray]: number of crossings in [0,1] per ray (no reference counterpart)."""
+    name = "count_crossings"
+    nd = len(shape)
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, nd)
+    _affine(name, M_inv, b, nd)
+    out = torch.empty((n_ray,), device=src.device, dtype=torch.int32)
+    with torch.cuda.device(src.device):
+        if nd == 3:
+            N.call("ctlab_count_crossings_3d", *[int(s) for s in shape], N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
+                   N.ptr(b), zero_rule, N.ptr(out), N.stream_of(src))
+        else:
+            N.call("ctlab_count_crossings_2d", *[int(s) for s in shape], N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
+                   N.ptr(b), zero_rule, N.ptr(out), N.stream_of(src))
+    return out
+
+
+def _tvals(name, tvals, n_ray):
+    _check_f32(name, tvals)
+    _require(tvals.dim() == 2 and tvals.shape[0] == n_ray, f"{name}: tvals must be [n_ray, K]")
+    return int(tvals.shape[1])
+
+
+def forward_project_2d_cuda(img, tvals, src, dst, M_inv, b):
+    """[B, n_ray] line integrals from a crossing table (src/ct_projector_2d.cu:179-200)."""
+    name = "forward_project_2d_cuda"
+    _check_f32(name, img, src, dst, M_inv, b)
+    batch, (nr, nc) = _vol_dims(name, img, 2)
+    n_ray = _rays(name, src, dst, 2)
+    K = _tvals(name, tvals, n_ray)
+    _affine(name, M_inv, b, 2)
+    out = torch.empty((batch, n_ray), device=img.device, dtype=torch.float32)
+    with torch.cuda.device(img.device):
+        N.call("ctlab_forward_project_2d_tvals", N.ptr(img), batch, nr, nc, N.ptr(tvals), K, N.ptr(src), N.ptr(dst),
+               n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out), N.stream_of(img))
+    return out
+
+
+def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col):
+    """[B, n_row, n_col] back projection from a crossing table (src/ct_projector_2d.cu:202-221)."""
+    name = "back_project_2d_cuda"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
+    K = _tvals(name, tvals, n_ray)
+    _affine(name, M_inv, b, 2)
+    out = torch.empty((batch, int(n_row), int(n_col)), device=sino.device, dtype=torch.float32)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_2d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), int(n_row), int(n_col), 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b):
+    """[B, n_ray] line integrals from a crossing table (src/ct_projector_3d.cu:403-433)."""
+    name = "forward_project_3d_cuda"
+    _check_f32(name, volume, src, dst, M_inv, b)
+    batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
+    n_ray = _rays(name, src, dst, 3)
+    K = _tvals(name, tvals, n_ray)
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    with torch.cuda.device(volume.device):
+        N.call("ctlab_forward_project_3d_tvals", N.ptr(volume), batch, nx, ny, nz, N.ptr(tvals), K, N.ptr(src),
+               N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out), N.stream_of(volume))
+    return out
+
+
+def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z):
+    """[B, n_x, n_y, n_z] back projection from a crossing table (src/ct_projector_3d.cu:435-462)."""
+    name = "back_project_3d_cuda"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    K = _tvals(name, tvals, n_ray)
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# uint16-compressed tables (bindings.cpp:90-95)
+# ---------------------------------------------------------------------------
+
+
+def compress_tvals_3d_cuda(tvals, src, dst, voxel_diag_len, *, semantics: int = N.COMPRESS_CUDA):
+    """(int16 [n_ray, K] holding uint16 deltas, start f32 [n_ray], scale f32 [n_ray])
+    (src/ct_projector_3d.cu:464-489)."""
+    name = "compress_tvals_3d_cuda"
+    _check_f32(name, tvals, src, dst)
+    n_ray = _rays(name, src, dst, 3)
+    K = _tvals(name, tvals, n_ray)
+    dev = tvals.device
+    u16 = torch.empty((n_ray, K), device=dev, dtype=torch.int16)
+    start = torch.empty((n_ray,), device=dev, dtype=torch.float32)
+    scale = torch.empty((n_ray,), device=dev, dtype=torch.float32)
+    if n_ray and K:
+        with torch.cuda.device(dev):
+            N.call("ctlab_compress_tvals_3d", N.ptr(tvals), n_ray, K, N.ptr(src), N.ptr(dst), float(voxel_diag_len),
+                   semantics, N.ptr(u16), N.ptr(start), N.ptr(scale), N.stream_of(tvals))
+    return u16, start, scale
+
+
+def _u16(name, t_uint16, t_start, t_scale, n_ray):
+    _require(isinstance(t_uint16, torch.Tensor) and t_uint16.dtype in (torch.int16, torch.uint16),
+             f"{name}: t_uint16 must be int16 (uint16 bits)")
+    _require(t_uint16.is_cuda and t_uint16.is_contiguous(), f"{name}: t_uint16 must be a contiguous CUDA tensor")
+    _require(t_uint16.dim() == 2 and t_uint16.shape[0] == n_ray, f"{name}: t_uint16 must be [n_ray, K]")
+    _check_f32(name, t_start, t_scale)
+    _require(t_start.numel() == n_ray and t_scale.numel() == n_ray, f"{name}: start/scale must be [n_ray]")
+    return int(t_uint16.shape[1])
+
+
+def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, dst, M_inv, b):
+    """[B, n_ray] line integrals from a uint16 table (src/ct_projector_3d.cu:491-522)."""
+    name = "forward_project_3d_compressed_cuda"
+    _check_f32(name, volume, src, dst, M_inv, b)
+    batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
+    n_ray = _rays(name, src, dst, 3)
+    K = _u16(name, t_uint16, t_start, t_scale, n_ray)
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    with torch.cuda.device(volume.device):
+        N.call("ctlab_forward_project_3d_u16", N.ptr(volume), batch, nx, ny, nz, N.ptr(t_uint16), N.ptr(t_start),
+               N.ptr(t_scale), K, N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out),
+               N.stream_of(volume))
+    return out
+
+
+def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, M_inv, b, n_x, n_y, n_z):
+    """[B, n_x, n_y, n_z] back projection from a uint16 table (src/ct_projector_3d.cu:524-553)."""
+    name = "back_project_3d_compressed_cuda"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    K = _u16(name, t_uint16, t_start, t_scale, n_ray)
+    _affine(name, M_inv, b, 3)
+    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_u16", N.ptr(sino), batch, N.ptr(t_uint16), N.ptr(t_start), N.ptr(t_scale), K,
+               N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out),
+               N.stream_of(sino))
+    return out
+
+
+REFERENCE_NAMES = (
+    "compute_intersections_2d", "forward_project_2d_cuda", "back_project_2d_cuda",
+    "forward_project_2d_on_the_fly_cuda", "back_project_2d_on_the_fly_cuda",
+    "compute_intersections_3d", "forward_project_3d_cuda", "back_project_3d_cuda",
+    "compress_tvals_3d_cuda", "forward_project_3d_compressed_cuda", "back_project_3d_compressed_cuda",
+    "forward_project_3d_on_the_fly_cuda", "back_project_3d_on_the_fly_cuda",
+)

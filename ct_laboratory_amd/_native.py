@@ -27,6 +27,8 @@ ZERO_THRESH = 0
 ZERO_EXACT = 1
 
 MODES = {"torch": MODE_TORCH, "otf": MODE_OTF}
+COMPRESS_CUDA = 0
+COMPRESS_TORCH = 1
 
 # name -> argtypes (restype is int status unless listed in _RESTYPES)
 _SIGS = {
@@ -37,6 +39,18 @@ _SIGS = {
     "ctlab_back_project_2d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _INT, _INT, _INT, _P, _P],
     "ctlab_trace_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
     "ctlab_trace_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
+    "ctlab_compute_intersections_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _P, _P],
+    "ctlab_compute_intersections_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _P, _P],
+    "ctlab_count_crossings_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _P, _P],
+    "ctlab_count_crossings_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _P, _P],
+    "ctlab_forward_project_2d_tvals": [_P, _I64, _I64, _I64, _P, _I64, _P, _P, _I64, _P, _P, _P, _P],
+    "ctlab_back_project_2d_tvals": [_P, _I64, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _INT, _P, _P],
+    "ctlab_forward_project_3d_tvals": [_P, _I64, _I64, _I64, _I64, _P, _I64, _P, _P, _I64, _P, _P, _P, _P],
+    "ctlab_back_project_3d_tvals": [_P, _I64, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P, _P],
+    "ctlab_compress_tvals_3d": [_P, _I64, _I64, _P, _P, ctypes.c_float, _INT, _P, _P, _P, _P],
+    "ctlab_forward_project_3d_u16": [_P, _I64, _I64, _I64, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _P, _P],
+    "ctlab_back_project_3d_u16": [_P, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P,
+                                  _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }

@@ -317,4 +317,100 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
     }
 }
 
+// Emits, in increasing order, every plane crossing t in [0, 1] of the whole
+// ray: the sorted list of compute_intersections_{2d,3d}
+// (ct_projector_3d_torch.py:78-158, src/ct_projector_3d.cu:42-96) without its
+// INF padding.  Crossings use the division form RN(RN(p-0.5-s)/dir); an axis
+// has none when the zero rule says so.  Returns the number emitted.
+template <int ND, class Emit>
+__host__ __device__ __forceinline__ int emit_crossings(const Grid<ND>& g, const float* src, const float* dst,
+                                                       Emit&& emit) {
+    float sv[ND], dv[ND], dir[ND], next[ND];
+    int pl[ND], step[ND];
+    const float kInf = __builtin_huge_valf();
+    to_voxel<ND, MODE_TORCH>(g, src, sv);
+    to_voxel<ND, MODE_TORCH>(g, dst, dv);
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        dir[a] = dv[a] - sv[a];
+        next[a] = kInf;
+        pl[a] = 0;
+        step[a] = 0;
+        const bool active = (g.zero_rule == ZERO_EXACT) ? (dir[a] != 0.0f) : !(fabsf(dir[a]) < 1.0e-12f);
+        if (!active) continue;
+        const int n = g.n[a];
+        int p;
+        // First crossing with t >= 0 in traversal order (monotone in p).
+        if (dir[a] > 0.0f) {
+            step[a] = 1;
+            const float f = ceilf(sv[a] + 0.5f);
+            p = (f >= 0.0f) ? ((f <= (float)(n + 1)) ? (int)f : n + 1) : 0;
+            while (p > 0 && crossing<MODE_TORCH>(p - 1, sv[a], dir[a]) >= 0.0f) --p;
+            while (p <= n && crossing<MODE_TORCH>(p, sv[a], dir[a]) < 0.0f) ++p;
+        } else {
+            step[a] = -1;
+            const float f = floorf(sv[a] + 0.5f);
+            p = (f >= -1.0f) ? ((f <= (float)n) ? (int)f : n) : -1;
+            while (p < n && crossing<MODE_TORCH>(p + 1, sv[a], dir[a]) >= 0.0f) ++p;
+            while (p >= 0 && crossing<MODE_TORCH>(p, sv[a], dir[a]) < 0.0f) --p;
+        }
+        pl[a] = p;
+        if (p >= 0 && p <= n) {
+            const float t = crossing<MODE_TORCH>(p, sv[a], dir[a]);
+            next[a] = (t > 1.0f) ? kInf : t;
+        }
+    }
+    int count = 0;
+    while (true) {
+        float m = next[0];
+#pragma unroll
+        for (int a = 1; a < ND; ++a) m = fminf(m, next[a]);
+        if (!(m <= 1.0f)) break;
+        int ax = ND - 1;
+#pragma unroll
+        for (int a = ND - 2; a >= 0; --a)
+            if (next[a] == m) ax = a;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+            if (a == ax) {
+                const int p = pl[a] + step[a];
+                pl[a] = p;
+                float tn = kInf;
+                if (p >= 0 && p <= g.n[a]) {
+                    tn = crossing<MODE_TORCH>(p, sv[a], dir[a]);
+                    if (tn > 1.0f) tn = kInf;
+                }
+                next[a] = tn;
+            }
+        }
+        emit(m);
+        ++count;
+    }
+    return count;
+}
+
+// Segment geometry of one pair of crossings given as t values, in the op order
+// of the torch consumer (3d_torch.py:219-271): world points src + t (dst-src),
+// length |P1 - P0|, midpoint voxel by round-half-even.  p0 holds P(t0) on
+// entry and P(t1) on return.  Returns the flat voxel or -1; *len receives the
+// length.
+template <int ND, bool DIAG>
+__host__ __device__ __forceinline__ int segment_from_t(const Grid<ND>& g, const float* src, const float* vw,
+                                                       float* p0, float t1, float* len) {
+    float p1[ND], m[ND], c[ND];
+    float ss = 0.0f;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        p1[a] = src[a] + t1 * vw[a];
+        const float dd = p1[a] - p0[a];
+        const float sq = dd * dd;
+        ss = (a == 0) ? sq : ss + sq;
+        m[a] = 0.5f * (p0[a] + p1[a]);
+        p0[a] = p1[a];
+    }
+    *len = __builtin_sqrtf(ss);
+    mid_coords<ND, MODE_TORCH, DIAG>(g, m, c);
+    return flat_voxel<ND>(g, c);
+}
+
 }  // namespace ctlab

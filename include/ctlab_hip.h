@@ -107,6 +107,76 @@ int ctlab_trace_2d(int64_t n_row, int64_t n_col, const float* src, const float* 
                    const float* M_inv, const float* b, int mode, int zero_rule, int64_t cap, int32_t* idx,
                    float* len, int32_t* counts, ctlab_stream_t stream);
 
+
+/* ---------------------------------------------------------------------- */
+/* Crossing tables                                                         */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces _C.compute_intersections_2d / _3d (bindings.cpp:67-68, 83-84;
+ * src/ct_projector_2d.cu:158-177, src/ct_projector_3d.cu:381-401): out [n_ray, K]
+ * receives each ray's crossings t in [0,1] in increasing order, then INF.
+ * K = (n_row+1)+(n_col+1) (resp. (n_x+1)+(n_y+1)+(n_z+1)) reproduces the
+ * reference's full table; a smaller K truncates (rows with more crossings lose
+ * their tail).  zero_rule: CTLAB_ZERO_THRESH as the CUDA kernels,
+ * CTLAB_ZERO_EXACT as compute_intersections_2d_torch (2d_torch.py:42). */
+int ctlab_compute_intersections_2d(int64_t n_row, int64_t n_col, const float* src, const float* dst,
+                                   int64_t n_ray, const float* M_inv, const float* b, int zero_rule, int64_t K,
+                                   float* out, ctlab_stream_t stream);
+int ctlab_compute_intersections_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* src, const float* dst,
+                                   int64_t n_ray, const float* M_inv, const float* b, int zero_rule, int64_t K,
+                                   float* out, ctlab_stream_t stream);
+
+/* Number of crossings per ray (int32 [n_ray]), for sizing trimmed tables (the
+ * reference trims all-INF columns on the host, 3d_module.py:92-98). */
+int ctlab_count_crossings_2d(int64_t n_row, int64_t n_col, const float* src, const float* dst, int64_t n_ray,
+                             const float* M_inv, const float* b, int zero_rule, int32_t* counts,
+                             ctlab_stream_t stream);
+int ctlab_count_crossings_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* src, const float* dst,
+                             int64_t n_ray, const float* M_inv, const float* b, int zero_rule, int32_t* counts,
+                             ctlab_stream_t stream);
+
+/* Replaces _C.forward_project_2d_cuda (bindings.cpp:69-70; src/ct_projector_2d.cu:179-200):
+ * tvals [n_ray, K] sorted crossings; pairs with an INF end are skipped. */
+int ctlab_forward_project_2d_tvals(const float* image, int64_t batch, int64_t n_row, int64_t n_col,
+                                   const float* tvals, int64_t K, const float* src, const float* dst, int64_t n_ray,
+                                   const float* M_inv, const float* b, float* out, ctlab_stream_t stream);
+/* Replaces _C.back_project_2d_cuda (bindings.cpp:71-72; src/ct_projector_2d.cu:202-221). */
+int ctlab_back_project_2d_tvals(const float* sino, int64_t batch, const float* tvals, int64_t K, const float* src,
+                                const float* dst, int64_t n_ray, const float* M_inv, const float* b, int64_t n_row,
+                                int64_t n_col, int accumulate, float* out, ctlab_stream_t stream);
+/* Replaces _C.forward_project_3d_cuda (bindings.cpp:85-86; src/ct_projector_3d.cu:403-433). */
+int ctlab_forward_project_3d_tvals(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
+                                   const float* tvals, int64_t K, const float* src, const float* dst, int64_t n_ray,
+                                   const float* M_inv, const float* b, float* out, ctlab_stream_t stream);
+/* Replaces _C.back_project_3d_cuda (bindings.cpp:87-88; src/ct_projector_3d.cu:435-462). */
+int ctlab_back_project_3d_tvals(const float* sino, int64_t batch, const float* tvals, int64_t K, const float* src,
+                                const float* dst, int64_t n_ray, const float* M_inv, const float* b, int64_t n_x,
+                                int64_t n_y, int64_t n_z, int accumulate, float* out, ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* uint16-compressed tables                                                */
+/* ---------------------------------------------------------------------- */
+
+#define CTLAB_COMPRESS_CUDA 0  /* compress_tvals_3d_kernel: roundf, INF -> delta 0 (3d.cu:160-212) */
+#define CTLAB_COMPRESS_TORCH 1 /* compress_tvals_to_uint16: round-half-even, INF -> 10.0 (3d_torch.py:4-65) */
+
+/* Replaces _C.compress_tvals_3d_cuda (bindings.cpp:90-91; src/ct_projector_3d.cu:464-489):
+ * tvals [n_ray, K] -> u16 [n_ray, K] (delta counts, first 0), start, scale [n_ray]. */
+int ctlab_compress_tvals_3d(const float* tvals, int64_t n_ray, int64_t K, const float* src, const float* dst,
+                            float voxel_diag_len, int semantics, uint16_t* u16, float* start, float* scale,
+                            ctlab_stream_t stream);
+/* Replaces _C.forward_project_3d_compressed_cuda (bindings.cpp:92-93;
+ * src/ct_projector_3d.cu:491-522): t_k = start + (sum of deltas) * scale. */
+int ctlab_forward_project_3d_u16(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
+                                 const uint16_t* u16, const float* start, const float* scale, int64_t K,
+                                 const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                 const float* b, float* out, ctlab_stream_t stream);
+/* Replaces _C.back_project_3d_compressed_cuda (bindings.cpp:94-95; src/ct_projector_3d.cu:524-553). */
+int ctlab_back_project_3d_u16(const float* sino, int64_t batch, const uint16_t* u16, const float* start,
+                              const float* scale, int64_t K, const float* src, const float* dst, int64_t n_ray,
+                              const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
+                              int accumulate, float* out, ctlab_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

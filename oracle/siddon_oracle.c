@@ -595,7 +595,8 @@ int orc_compress_cuda(const float* tvals, int64_t n_ray, int64_t K, const float*
             float tv = t[i];
             if (isinf(tv)) { o[i] = 0; continue; }
             float q = roundf((tv - st) * inv);
-            int64_t kc = (int64_t)(uint32_t)(int64_t)q; /* (unsigned int) conversion */
+            /* (unsigned int) conversion as the GPU performs it: saturating */
+            int64_t kc = (q < 0.0f) ? 0 : (q >= 4294967296.0f) ? (int64_t)0xffffffffu : (int64_t)(uint32_t)q;
             int64_t diff = (int64_t)(int32_t)kc - (int64_t)(int32_t)kprev;
             if (diff < 0) diff = 0;
             if (diff > 32000) diff = 32000;
@@ -616,7 +617,8 @@ int orc_compress_torch(const float* tvals, int64_t n_ray, int64_t K, const float
         const float* s = src + 3 * r;
         const float* d = dst + 3 * r;
         float vx = d[0] - s[0], vy = d[1] - s[1], vz = d[2] - s[2];
-        float rl = sqrtf((vx * vx + vy * vy) + vz * vz);
+        /* torch.norm reduces float32 with a double accumulator */
+        float rl = (float)sqrt((double)vx * vx + (double)vy * vy + (double)vz * vz);
         if (rl < 1e-6f) rl = 1e-6f;
         float sc = (voxel_diag_len / 10000.0f) / rl;
         scale[r] = sc;

@@ -1,0 +1,135 @@
+"""GPU parity of the crossing-table paths: compute_intersections_{2d,3d},
+float-table and uint16-table projectors, compress_tvals_3d."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from ct_laboratory_amd import _C
+from ct_laboratory_amd import _native as N
+from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+from helpers import lattice_rays, random_geometry, random_rays, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
+
+
+def _same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and (a.view(np.int32) == b.view(np.int32)).all()
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("zero_rule", [N.ZERO_THRESH, N.ZERO_EXACT])
+def test_intersections_bit_exact(dev, nd, zero_rule):
+    rng = np.random.default_rng(20 + nd)
+    zr = "exact" if zero_rule == N.ZERO_EXACT else "thresh"
+    for trial in range(4):
+        shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial == 3))
+        src, dst = random_rays(rng, shape, s, 2000, ["far", "inside", "lattice", "far"][trial])
+        fn = _C.compute_intersections_3d if nd == 3 else _C.compute_intersections_2d
+        got = fn(*shape, _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev), zero_rule=zero_rule).cpu().numpy()
+        ref = O.intersections(shape, minv, b, src, dst, zero_rule=zr)
+        # -0.0 and +0.0 compare equal and may sort either way; compare as values there.
+        assert got.shape == ref.shape
+        assert _same_bits(got + 0.0, ref + 0.0)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_table_consumer_equals_on_the_fly(dev, nd):
+    """Full table + table projector == on-the-fly projector (torch semantics),
+    bit for bit: same segments, same accumulation order."""
+    rng = np.random.default_rng(30 + nd)
+    shape, M, minv, b, s = random_geometry(rng, nd, max_n=30)
+    src, dst = random_rays(rng, shape, s, 4000, "far")
+    zr = N.ZERO_THRESH
+    ts, td, tm, tb = _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev)
+    vol = _t(rng.uniform(0, 1, (2,) + shape), dev)
+    sino = _t(rng.uniform(0, 1, (2, src.shape[0])), dev)
+    if nd == 3:
+        tab = _C.compute_intersections_3d(*shape, ts, td, tm, tb, zero_rule=zr)
+        f_tab = _C.forward_project_3d_cuda(vol, tab, ts, td, tm, tb)
+        f_otf = _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, tm, tb, mode=N.MODE_TORCH)
+        b_tab = _C.back_project_3d_cuda(sino, tab, ts, td, tm, tb, *shape)
+        b_otf = _C.back_project_3d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH)
+    else:
+        tab = _C.compute_intersections_2d(*shape, ts, td, tm, tb, zero_rule=zr)
+        f_tab = _C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb)
+        f_otf = _C.forward_project_2d_on_the_fly_cuda(vol, ts, td, tm, tb, mode=N.MODE_TORCH, zero_rule=zr)
+        b_tab = _C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape)
+        b_otf = _C.back_project_2d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH, zero_rule=zr)
+    assert torch.equal(f_tab, f_otf)
+    assert rel_err(b_tab.cpu().numpy(), b_otf.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_table_projectors_vs_oracle(dev, nd):
+    rng = np.random.default_rng(40 + nd)
+    shape, M, minv, b, s = random_geometry(rng, nd, max_n=30)
+    src, dst = lattice_rays(rng, nd, 3000) if nd == 2 else random_rays(rng, shape, s, 3000, "inside")
+    if nd == 2:
+        shape, minv, b = (13, 11), np.eye(2, dtype=np.float32), np.zeros(2, np.float32)
+    zr = "exact" if nd == 2 else "thresh"
+    tab = O.intersections(shape, minv, b, src, dst, zero_rule=zr)
+    vol = rng.uniform(0, 1, (2,) + shape).astype(np.float32)
+    sino = rng.uniform(0, 1, (2, src.shape[0])).astype(np.float32)
+    ts, td, tm, tb, tt = _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev), _t(tab, dev)
+    if nd == 3:
+        f = _C.forward_project_3d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
+        bk = _C.back_project_3d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
+    else:
+        f = _C.forward_project_2d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
+        bk = _C.back_project_2d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
+    assert rel_err(f.cpu().numpy(), O.forward_tvals(vol, tab, minv, b, src, dst)) < 1e-5
+    assert rel_err(bk.cpu().numpy(), O.back_tvals(sino, tab, shape, minv, b, src, dst)) < 1e-5
+
+
+@pytest.mark.parametrize("semantics,name", [(N.COMPRESS_CUDA, "cuda"), (N.COMPRESS_TORCH, "torch")])
+def test_compress_bit_exact(dev, semantics, name):
+    n = 40
+    src, dst = build_conebeam_3d_geometry(9, 33, 21, 1000.0, 1500.0, det_spacing=3.5)
+    b = np.full(3, -(n - 1) / 2 * 1.25, np.float32)
+    M = np.eye(3, dtype=np.float32) * np.float32(1.25)
+    minv = np.linalg.inv(M.astype(np.float64)).astype(np.float32)
+    tab = O.intersections((n, n, n), minv, b, src.numpy(), dst.numpy())
+    vdl = O.voxel_diag_len(M)
+    u, st, sc = _C.compress_tvals_3d_cuda(_t(tab, dev), src.to(dev), dst.to(dev), float(vdl), semantics=semantics)
+    ref = (O.compress_cuda if name == "cuda" else O.compress_torch)(tab, src.numpy(), dst.numpy(), vdl)
+    assert (u.cpu().numpy().view(np.uint16) == ref[0].view(np.uint16)).all()
+    assert _same_bits(st.cpu().numpy(), ref[1]) and _same_bits(sc.cpu().numpy(), ref[2])
+
+
+def test_compressed_projectors_vs_oracle(dev):
+    n = 40
+    src, dst = build_conebeam_3d_geometry(9, 33, 21, 1000.0, 1500.0, det_spacing=3.5)
+    b = np.full(3, -(n - 1) / 2 * 1.25, np.float32)
+    M = np.eye(3, dtype=np.float32) * np.float32(1.25)
+    minv = np.linalg.inv(M.astype(np.float64)).astype(np.float32)
+    s_np, d_np = src.numpy(), dst.numpy()
+    tab = O.intersections((n, n, n), minv, b, s_np, d_np)
+    u, st, sc = O.compress_cuda(tab, s_np, d_np, O.voxel_diag_len(M))
+    rng = np.random.default_rng(3)
+    vol = rng.uniform(0, 1, (2, n, n, n)).astype(np.float32)
+    sino = rng.uniform(0, 1, (2, s_np.shape[0])).astype(np.float32)
+    tu = torch.from_numpy(u.view(np.int16)).to(dev)
+    ts, td, tm, tb = src.to(dev), dst.to(dev), _t(minv, dev), _t(b, dev)
+    f = _C.forward_project_3d_compressed_cuda(_t(vol, dev), tu, _t(st, dev), _t(sc, dev), ts, td, tm, tb)
+    bk = _C.back_project_3d_compressed_cuda(_t(sino, dev), tu, _t(st, dev), _t(sc, dev), ts, td, tm, tb, n, n, n)
+    assert rel_err(f.cpu().numpy(), O.forward_u16(vol, u, st, sc, minv, b, s_np, d_np)) < 1e-5
+    assert rel_err(bk.cpu().numpy(), O.back_u16(sino, u, st, sc, (n, n, n), minv, b, s_np, d_np)) < 1e-5
+    # Against the uncompressed table: decode error is at most half a quantum per crossing.
+    f_full = O.forward_tvals(vol, tab, minv, b, s_np, d_np)
+    assert rel_err(f.cpu().numpy(), f_full) < 2e-3
+
+
+def test_count_crossings(dev):
+    rng = np.random.default_rng(9)
+    shape, M, minv, b, s = random_geometry(rng, 3)
+    src, dst = random_rays(rng, shape, s, 2000, "far")
+    cnt = _C.count_crossings(shape, _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev)).cpu().numpy()
+    ref = np.isfinite(O.intersections(shape, minv, b, src, dst)).sum(axis=1)
+    assert (cnt == ref).all()
