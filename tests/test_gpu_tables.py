@@ -121,9 +121,14 @@ def test_compressed_projectors_vs_oracle(dev):
     bk = _C.back_project_3d_compressed_cuda(_t(sino, dev), tu, _t(st, dev), _t(sc, dev), ts, td, tm, tb, n, n, n)
     assert rel_err(f.cpu().numpy(), O.forward_u16(vol, u, st, sc, minv, b, s_np, d_np)) < 1e-5
     assert rel_err(bk.cpu().numpy(), O.back_u16(sino, u, st, sc, (n, n, n), minv, b, s_np, d_np)) < 1e-5
-    # Against the uncompressed table: decode error is at most half a quantum per crossing.
+    # Against the uncompressed table: decode error is at most half a quantum per
+    # crossing, on rows where no delta hit the 32000 clamp.  (Rays crossing
+    # planes far outside the grid, e.g. z-planes along a whole low-slope
+    # cone-beam ray, clamp and then lag: that is the reference's format.)
     f_full = O.forward_tvals(vol, tab, minv, b, s_np, d_np)
-    assert rel_err(f.cpu().numpy(), f_full) < 2e-3
+    ok = (u.max(axis=1) < 32000) & (f_full[0] > 0)
+    assert ok.sum() > 100
+    assert rel_err(f.cpu().numpy()[:, ok], f_full[:, ok]) < 2e-3
 
 
 def test_count_crossings(dev):
