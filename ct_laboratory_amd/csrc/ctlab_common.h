@@ -34,6 +34,16 @@ inline unsigned grid_for(int64_t n, int block = kBlock) { return (unsigned)((n +
 constexpr int64_t kMaxVoxels = 0x7fffffffLL;
 constexpr int64_t kMaxGridX = 0x7fffffffLL;
 
+// The traversal forms flat voxel indices with 24-bit multiply-adds: every grid
+// dimension, and n_y*n_z for 3D grids, must be below 2^24.
+inline int check_index_limits(const char* fn, const int64_t* n, int nd) {
+    for (int a = 0; a < nd; ++a)
+        if (n[a] >= (1LL << 24)) return fail(CTLAB_EINVAL, "%s: grid dimension %d >= 2^24", fn, a);
+    if (nd == 3 && n[1] * n[2] >= (1LL << 24))
+        return fail(CTLAB_EINVAL, "%s: n_y*n_z = %lld must be below 2^24", fn, (long long)(n[1] * n[2]));
+    return CTLAB_OK;
+}
+
 // Fills n floats with value on the stream (a kernel, never hipMemsetAsync, so
 // that the call is safe inside a captured graph).
 int fill_f32(float* p, int64_t n, float value, hipStream_t stream);

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict_
     load_ray<ND>(src, dst, r, s, d);
     const float* __restrict__ v = vol + bi * vol_elems;
     float acc = 0.0f;
-    auto gather = [&](int idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
+    auto gather = [&](uint32_t idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
     if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, gather);
     else
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_back_otf(const float* __restrict__ s
     float s[ND], d[ND];
     load_ray<ND>(src, dst, r, s, d);
     float* __restrict__ o = out + bi * vol_elems;
-    auto scatter = [&](int idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
+    auto scatter = [&](uint32_t idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
     if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, scatter);
     else
@@ -124,9 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(int64_t n_ray, Dims<ND> dims, 
     int32_t* io = idx_out + r * cap;
     float* lo = len_out + r * cap;
     int64_t k = 0;
-    auto record = [&](int idx, float len) {
+    auto record = [&](uint32_t idx, float len) {
         if (k < cap) {
-            io[k] = idx;
+            io[k] = (int32_t)idx;
             lo[k] = len;
         }
         ++k;
@@ -152,6 +152,8 @@ static int check_common(const char* fn, const int64_t* n, int64_t batch, int64_t
         vox *= n[a];
         CTLAB_REQUIRE(vox <= kMaxVoxels, "%s: grid has more than 2^31-1 voxels", fn);
     }
+    int st = check_index_limits(fn, n, ND);
+    if (st) return st;
     CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range [1, 65535]", fn, (long long)batch);
     CTLAB_REQUIRE(n_ray >= 0, "%s: n_ray %lld < 0", fn, (long long)n_ray);
     CTLAB_REQUIRE(grid_for(n_ray) <= kMaxGridX, "%s: too many rays", fn);

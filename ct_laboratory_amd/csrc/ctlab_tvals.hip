@@ -86,7 +86,9 @@ __device__ __forceinline__ void walk_table(const Grid<ND>& g, const float* s, co
     float vw[ND], p0[ND];
 #pragma unroll
     for (int a = 0; a < ND; ++a) vw[a] = d[a] - s[a];
-    if (K < 2) return;
+    if (K < 2 || !g.finite) return;
+    MidIndex<ND, DIAG> mi;
+    mi.init(g);
     float t0 = row[0];
 #pragma unroll
     for (int a = 0; a < ND; ++a) p0[a] = s[a] + t0 * vw[a];
@@ -100,9 +102,10 @@ __device__ __forceinline__ void walk_table(const Grid<ND>& g, const float* s, co
             continue;
         }
         float len;
-        const int idx = segment_from_t<ND, DIAG>(g, s, vw, p0, t1, &len);
+        uint32_t idx;
+        const bool in = segment_from_t<ND, DIAG>(g, mi, s, vw, p0, t1, &idx, &len);
         t0 = t1;
-        if (idx >= 0 && len > 0.0f) visit(idx, len);
+        if (in && len > 0.0f) visit(idx, len);
     }
 }
 
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_tvals(const float* __restric
     load_ray_t<ND>(src, dst, r, s, d);
     const float* __restrict__ v = vol + bi * vol_elems;
     float acc = 0.0f;
-    auto gather = [&](int idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
+    auto gather = [&](uint32_t idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
     if (g.diag)
         walk_table<ND, true>(g, s, d, tvals + r * K, K, gather);
     else
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_back_tvals(const float* __restrict__
     float s[ND], d[ND];
     load_ray_t<ND>(src, dst, r, s, d);
     float* __restrict__ o = out + bi * vol_elems;
-    auto scatter = [&](int idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
+    auto scatter = [&](uint32_t idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
     if (g.diag)
         walk_table<ND, true>(g, s, d, tvals + r * K, K, scatter);
     else
@@ -227,15 +230,19 @@ __device__ __forceinline__ void walk_u16(const Grid<3>& g, const float* s, const
         vw[a] = d[a] - s[a];
         p0[a] = s[a] + st * vw[a];
     }
+    if (!g.finite) return;
+    MidIndex<3, DIAG> mi;
+    mi.init(g);
     uint32_t ks = 0;
     for (int64_t i = 1; i < K; ++i) {
         ks += (uint32_t)row[i];
         // t1 = start + k * scale with the integer running sum k (3d.cu:128-129)
         const float t1 = __builtin_fmaf((float)ks, sc, st);
         float len;
-        const int idx = segment_from_t<3, DIAG>(g, s, vw, p0, t1, &len);
+        uint32_t idx;
+        const bool in = segment_from_t<3, DIAG>(g, mi, s, vw, p0, t1, &idx, &len);
         if (!(len >= 1e-9f)) continue;  // 3d.cu:137
-        if (idx >= 0) visit(idx, len);
+        if (in) visit(idx, len);
     }
 }
 
@@ -255,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_u16(const float* __restrict_
     load_ray_t<3>(src, dst, r, s, d);
     const float* __restrict__ v = vol + bi * vol_elems;
     float acc = 0.0f;
-    auto gather = [&](int idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
+    auto gather = [&](uint32_t idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
     if (g.diag)
         walk_u16<true>(g, s, d, u16 + r * K, K, start[r], scale[r], gather);
     else
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_back_u16(const float* __restrict__ s
     float s[3], d[3];
     load_ray_t<3>(src, dst, r, s, d);
     float* __restrict__ o = out + bi * vol_elems;
-    auto scatter = [&](int idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
+    auto scatter = [&](uint32_t idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
     if (g.diag)
         walk_u16<true>(g, s, d, u16 + r * K, K, start[r], scale[r], scatter);
     else
@@ -299,6 +306,8 @@ static int check_grid(const char* fn, const int64_t* n, int64_t* V) {
         vox *= n[a];
         CTLAB_REQUIRE(vox <= kMaxVoxels, "%s: grid has more than 2^31-1 voxels", fn);
     }
+    int st = check_index_limits(fn, n, ND);
+    if (st) return st;
     *V = vox;
     return CTLAB_OK;
 }

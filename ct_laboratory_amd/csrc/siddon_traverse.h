@@ -5,16 +5,16 @@
 // (ct_laboratory/tomography/ct_projector_3d_torch.py:78-158 and :161-274;
 // src/ct_projector_3d.cu:834-1012).  Here the per-axis crossing sequences are
 // each monotone in the plane index, so their merge *is* the sorted list: the
-// lane keeps one pending crossing per axis and pops the smallest, with no
-// scratch array and no sort.
+// lane keeps one pending crossing per axis in registers and pops the
+// smallest, with no scratch array and no sort.
 //
-// Bit-exactness.  Every crossing is evaluated with the reference's formula
+// Bit-exactness.  Every crossing equals the reference's value bit for bit
 // (torch: RN(RN(p-0.5-s)/dir); otf: RN(RN(p-0.5-s)*RN(1/dir))) and every
-// segment's voxel with the reference's midpoint rule and operation order, so
-// the (voxel, length) sequence equals the reference's for each segment.  The
-// file is compiled with floating-point contraction off; fused multiply-adds
-// appear only where written as __builtin_fmaf (the otf forms, which follow
-// nvcc's default contraction of the CUDA source).
+// segment's voxel is the reference's midpoint rule in the reference's
+// operation order, so the voxel sequence equals the reference's.  The file is
+// compiled with floating-point contraction off; fused multiply-adds appear only
+// where written as __builtin_fmaf (the exact-division correction and the otf
+// forms, which follow nvcc's default contraction of the CUDA source).
 //
 // Box clipping without losing exactness.  Only segments whose rounded
 // midpoint falls inside the grid contribute.  The lane jumps each axis to its
@@ -43,25 +43,29 @@ struct Grid {
     float minv[ND * ND];
     float b[ND];
     int diag;       // off-diagonal entries of minv are all zero
+    int finite;     // minv and b are finite
     int zero_rule;  // ZERO_THRESH: |dir| < 1e-12 has no crossings; ZERO_EXACT: dir == 0
 };
 
 template <int ND>
 __host__ __device__ __forceinline__ Grid<ND> load_grid(const int* n, const float* __restrict__ minv,
-                                              const float* __restrict__ b, int zero_rule) {
+                                                       const float* __restrict__ b, int zero_rule) {
     Grid<ND> g;
-    int diag = 1;
+    int diag = 1, fin = 1;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         g.n[a] = n[a];
         g.b[a] = b[a];
+        fin &= (fabsf(g.b[a]) <= 3.0e38f);
     }
 #pragma unroll
     for (int i = 0; i < ND * ND; ++i) {
         g.minv[i] = minv[i];
         if ((i / ND) != (i % ND) && g.minv[i] != 0.0f) diag = 0;
+        fin &= (fabsf(g.minv[i]) <= 3.0e38f);
     }
     g.diag = diag;
+    g.finite = fin;
     g.zero_rule = zero_rule;
     return g;
 }
@@ -90,7 +94,8 @@ __host__ __device__ __forceinline__ void to_voxel(const Grid<ND>& g, const float
     }
 }
 
-// Midpoint voxel coordinates (3d_torch.py:244-250; 3d.cu:985-991).
+// Midpoint voxel coordinates from the world midpoint (3d_torch.py:244-250;
+// 3d.cu:985-991).
 template <int ND, int MODE, bool DIAG>
 __host__ __device__ __forceinline__ void mid_coords(const Grid<ND>& g, const float* m, float* c) {
     float sh[ND];
@@ -120,34 +125,233 @@ __host__ __device__ __forceinline__ void mid_coords(const Grid<ND>& g, const flo
     }
 }
 
-// Round half to even and range check; returns the flat index or -1.
+// 24-bit multiply-add (v_mad_u32_u24 on the device).  Callers guarantee both
+// factors are below 2^24 (the host rejects grids with n_y*n_z >= 2^24).
+__host__ __device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b) + c;
+#else
+    return a * b + c;
+#endif
+}
+
+// Round half to even, range check and flat index ((i*n_y + j)*n_z + k as
+// i*(n_y*n_z) + (j*n_z + k)).  A NaN coordinate clamps to -1 (fmaxf returns the
+// non-NaN operand) and is out of range, as round(NaN).long() is in the
+// reference.
 template <int ND>
-__host__ __device__ __forceinline__ int flat_voxel(const Grid<ND>& g, const float* c) {
-    int idx = 0;
+__host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const float* c, uint32_t* idx) {
+    int32_t r[ND];
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        float r = __builtin_rintf(c[a]);  // v_rndne_f32: ties to even
-        ok = ok && (r >= 0.0f) && (r < (float)g.n[a]);
-        idx = idx * g.n[a] + (int)r;
+        const float f = __builtin_rintf(c[a]);  // v_rndne_f32: ties to even
+        // Clamp into int range first (the host conversion of out-of-range
+        // floats is undefined); every clamped value is out of the grid anyway.
+        const float fc = fminf(fmaxf(f, -1.0f), 2147483520.0f);
+        r[a] = (int32_t)fc;
+        ok = ok && ((uint32_t)r[a] < (uint32_t)g.n[a]);
     }
-    return ok ? idx : -1;
+    if (ND == 3) {
+        *idx = mad24((uint32_t)r[0], (uint32_t)(g.n[1] * g.n[2]), mad24((uint32_t)r[1], (uint32_t)g.n[2],
+                                                                       (uint32_t)r[ND - 1]));
+    } else {
+        *idx = mad24((uint32_t)r[0], (uint32_t)g.n[ND - 1], (uint32_t)r[ND - 1]);
+    }
+    return ok;
 }
 
-// Plane crossing of axis a at plane index p.
-template <int MODE>
-__host__ __device__ __forceinline__ float crossing(int p, float s, float dir_or_inv) {
-    float num = ((float)p - 0.5f) - s;
-    if (MODE == MODE_OTF) return num * dir_or_inv;
-    return num / dir_or_inv;
+// Same as flat_voxel for finite coordinates with |c| < 2^22, in two integer-
+// exact steps per axis: RN(c + 1.5*2^23) lies in [2^23, 2^24) where the float
+// spacing is 1, so the addition itself rounds c half-to-even, and the bit
+// pattern minus that of 1.5*2^23 is the rounded integer.  The clipped
+// traversal only evaluates midpoints within a few voxels of the grid (and the
+// grid dimensions are below 2^24 with E < 1), so |c| stays far below 2^22.
+template <int ND>
+__host__ __device__ __forceinline__ bool flat_voxel_bounded(const Grid<ND>& g, const float* c, uint32_t* idx) {
+    const float kMagic = 12582912.0f;  // 1.5 * 2^23
+    uint32_t r[ND];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float f = c[a] + kMagic;
+        r[a] = (uint32_t)(__builtin_bit_cast(int32_t, f) - 0x4b400000);
+        ok = ok && (r[a] < (uint32_t)g.n[a]);
+    }
+    if (ND == 3) {
+        *idx = mad24(r[0], (uint32_t)(g.n[1] * g.n[2]), mad24(r[1], (uint32_t)g.n[2], r[ND - 1]));
+    } else {
+        *idx = mad24(r[0], (uint32_t)g.n[ND - 1], r[ND - 1]);
+    }
+    return ok;
 }
+
+// Square root of a segment length.  The device uses the hardware
+// v_sqrt_f32 (within 1 ulp); the host build (test harness) uses the correctly
+// rounded sqrtf.  Lengths only scale contributions; voxel indices never depend
+// on them.
+__host__ __device__ __forceinline__ float seg_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+
+// Plane crossing with plane coordinate f = p - 0.5 (exact), voxel start s,
+// voxel direction d and y = RN(1/d):
+//   torch: RN(RN(f-s)/d), computed as q0 = RN(num*y); r = num - q0*d (exact,
+//          one fma); RN(q0 + r*y).  With y = RN(1/d) and no over/underflow this
+//          is the correctly rounded quotient (Markstein's theorem; also checked
+//          against IEEE division on 3.5e8 random and adversarial operand
+//          pairs), i.e. bit-identical to the reference's division.
+//   otf:   RN(RN(f-s)*y), the reference's reciprocal form (3d.cu:866-868).
+template <int MODE>
+__host__ __device__ __forceinline__ float crossing_f(float f, float s, float d, float y) {
+    const float num = f - s;
+    const float q0 = num * y;
+    if (MODE == MODE_OTF) return q0;
+    const float r = __builtin_fmaf(-q0, d, num);
+    return __builtin_fmaf(r, y, q0);
+}
+
+template <int MODE>
+__host__ __device__ __forceinline__ float crossing_p(int p, float s, float d, float y) {
+    return crossing_f<MODE>((float)p - 0.5f, s, d, y);
+}
+
+// First plane index p (in the axis' traversal direction) whose crossing is
+// >= t_lo, or one past the grid if none.  Crossings are monotone in p, so the
+// guess from the coordinate at t_lo is corrected by at most a few steps; the
+// clamps keep a NaN guess inside [-1, n+1] so both loops stay bounded.
+template <int MODE>
+__host__ __device__ __forceinline__ int first_plane(float s, float d, float y, int n, int step, float t_lo) {
+    const float c = s + t_lo * d;
+    int p;
+    if (step > 0) {
+        const float f = ceilf(c + 0.5f);
+        p = (f >= 0.0f) ? ((f <= (float)(n + 1)) ? (int)f : n + 1) : 0;
+        while (p > 0 && crossing_p<MODE>(p - 1, s, d, y) >= t_lo) --p;
+        while (p <= n && crossing_p<MODE>(p, s, d, y) < t_lo) ++p;
+    } else {
+        const float f = floorf(c + 0.5f);
+        p = (f >= -1.0f) ? ((f <= (float)n) ? (int)f : n) : -1;
+        while (p < n && crossing_p<MODE>(p + 1, s, d, y) >= t_lo) ++p;
+        while (p >= 0 && crossing_p<MODE>(p, s, d, y) < t_lo) --p;
+    }
+    return p;
+}
+
+// Merge of the per-axis crossing sequences, held in scalar registers (no
+// arrays, so nothing is dynamically indexed).  Axis a keeps the plane
+// coordinate f_a = p_a - 0.5 of its pending crossing, its step (+-1, 0 when
+// inactive), e_a = the plane coordinate one past its last plane, and the
+// pending crossing value n_a (INF when exhausted or past t = 1).
+template <int ND, int MODE>
+struct Merge {
+    float s0 = 0, s1 = 0, s2 = 0, d0 = 1, d1 = 1, d2 = 1, y0 = 1, y1 = 1, y2 = 1;
+    float f0 = 0, f1 = 0, f2 = 0, st0 = 0, st1 = 0, st2 = 0, e0 = 0, e1 = 0, e2 = 0;
+    float n0 = __builtin_huge_valf(), n1 = __builtin_huge_valf(), n2 = __builtin_huge_valf();
+
+    // Initialises axis a (a is a compile-time constant after unrolling).
+    __host__ __device__ __forceinline__ void set(int a, float s, float d, float y, int p, int step, int n,
+                                                 float t_next) {
+        const float f = (float)p - 0.5f;
+        const float e = (step > 0) ? (float)n + 0.5f : -1.5f;
+        const float st = (float)step;
+        if (a == 0) { s0 = s; d0 = d; y0 = y; f0 = f; st0 = st; e0 = e; n0 = t_next; }
+        if (a == 1) { s1 = s; d1 = d; y1 = y; f1 = f; st1 = st; e1 = e; n1 = t_next; }
+        if (a == 2) { s2 = s; d2 = d; y2 = y; f2 = f; st2 = st; e2 = e; n2 = t_next; }
+    }
+
+    // Advanced candidate of one axis: the crossing after its pending one.
+    __host__ __device__ __forceinline__ static float advance(float f, float st, float s, float d, float y, float e,
+                                                             float* f_out) {
+        const float fn = f + st;
+        const float tn = crossing_f<MODE>(fn, s, d, y);
+        *f_out = fn;
+        // Past the grid's last plane the axis is exhausted.  Values above 1 need
+        // no test: pop() only accepts crossings <= t_hi <= 1.
+        return (fn == e) ? __builtin_huge_valf() : tn;
+    }
+
+    // Pops the smallest pending crossing if it is <= t_hi.  Every axis
+    // evaluates its own candidate from its own registers and only the results
+    // are selected: selecting the operands instead lets the compiler turn the
+    // per-axis state into an indexed stack array (scratch memory).
+    __host__ __device__ __forceinline__ bool pop(float t_hi, float& t) {
+        const float m = (ND == 3) ? fminf(fminf(n0, n1), n2) : fminf(n0, n1);
+        if (!(m <= t_hi)) return false;
+        const bool k0 = (n0 == m);
+        const bool k1 = !k0 && (n1 == m);
+        float g0, g1, g2 = 0.0f;
+        const float c0 = advance(f0, st0, s0, d0, y0, e0, &g0);
+        const float c1 = advance(f1, st1, s1, d1, y1, e1, &g1);
+        f0 = k0 ? g0 : f0;
+        n0 = k0 ? c0 : n0;
+        if (ND == 3) {
+            const bool k2 = !k0 && !k1;
+            const float c2 = advance(f2, st2, s2, d2, y2, e2, &g2);
+            f1 = k1 ? g1 : f1;
+            n1 = k1 ? c1 : n1;
+            f2 = k2 ? g2 : f2;
+            n2 = k2 ? c2 : n2;
+        } else {
+            f1 = k0 ? f1 : g1;
+            n1 = k0 ? n1 : c1;
+        }
+        t = m;
+        return true;
+    }
+};
+
+// Midpoint voxel of a segment from its end points, folded: with S = P0+P1,
+// RN(Minv*RN(RN(0.5*S) - b)) == RN((0.5*Minv) * RN(S - 2b)) exactly (scaling
+// by a power of two commutes with rounding), so h = 0.5*Minv and b2 = 2b are
+// precomputed.  Diagonal grids use only h[a][a]: the off-diagonal products are
+// signed zeros and adding them changes neither value nor rounding.
+template <int ND, bool DIAG>
+struct MidIndex {
+    float h[ND * ND];
+    float b2[ND];
+    __host__ __device__ __forceinline__ void init(const Grid<ND>& g) {
+#pragma unroll
+        for (int i = 0; i < ND * ND; ++i) h[i] = 0.5f * g.minv[i];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) b2[a] = 2.0f * g.b[a];
+    }
+    __host__ __device__ __forceinline__ void from_ends(const float* p0, const float* p1, float* c) const {
+        float sh[ND];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) sh[a] = (p0[a] + p1[a]) - b2[a];
+        if (DIAG) {
+#pragma unroll
+            for (int a = 0; a < ND; ++a) c[a] = h[a * ND + a] * sh[a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < ND; ++a) {
+                float acc = h[a * ND] * sh[0];
+#pragma unroll
+                for (int j = 1; j < ND; ++j) acc = acc + h[a * ND + j] * sh[j];
+                c[a] = acc;
+            }
+        }
+    }
+};
 
 // Visits every segment of one ray whose rounded midpoint lies in the grid,
 // in increasing t, as visit(flat_index, length).  Zero-length segments are
 // not visited (they contribute exactly zero to both projections).
 template <int ND, int MODE, bool DIAG, class Visit>
 __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const float* src, const float* dst,
-                                          Visit&& visit) {
+                                                   Visit&& visit) {
+    if (!g.finite) return;  // the reference's indices are all out of range then
+    // Non-finite endpoints make every reference world point INF or NaN, and
+    // every such segment is masked (3d_torch.py:233-260): no contribution.
+    bool fin = true;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) fin = fin && (fabsf(src[a]) <= 3.0e38f) && (fabsf(dst[a]) <= 3.0e38f);
+    if (!fin) return;
     float vw[ND], sv[ND], dv[ND];
 #pragma unroll
     for (int a = 0; a < ND; ++a) vw[a] = dst[a] - src[a];
@@ -164,111 +368,73 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
     to_voxel<ND, MODE>(g, src, sv);
     to_voxel<ND, MODE>(g, dst, dv);
 
-    float dirq[ND];  // divisor (torch) or reciprocal (otf) per axis
-    float next[ND];
-    int pl[ND], step[ND];
-    bool act[ND];
+    float dir[ND], y[ND];
+    int step[ND];
     float t_lo = 0.0f, t_hi = 1.0f;
     const float kInf = __builtin_huge_valf();
 
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        const float dir = dv[a] - sv[a];
+        dir[a] = dv[a] - sv[a];
         // Rounding-error bound of a midpoint coordinate on this axis, voxel units.
         float mag = fabsf(sv[a]) + fabsf(dv[a]) + (float)g.n[a] + 1.0f;
 #pragma unroll
         for (int j = 0; j < ND; ++j)
             mag += fabsf(g.minv[a * ND + j]) * (fabsf(src[j]) + fabsf(dst[j]) + fabsf(g.b[j]));
         const float E = mag * 7.62939453125e-06f;  // 2^-17
+        // Midpoint coordinates stay within n + 5E of the grid; with E <= 8 they
+        // are far below the 2^22 bound of flat_voxel_bounded.  Endpoints that far
+        // out (~1e11 voxels) carry no usable float32 crossings anyway.
+        if (!(E <= 8.0f)) return;
         bool active;
         if (MODE == MODE_OTF) {
-            active = fabsf(dir) > 1.0e-12f;
+            active = fabsf(dir[a]) > 1.0e-12f;
         } else {
-            active = (g.zero_rule == ZERO_EXACT) ? (dir != 0.0f) : !(fabsf(dir) < 1.0e-12f);
+            active = (g.zero_rule == ZERO_EXACT) ? (dir[a] != 0.0f) : !(fabsf(dir[a]) < 1.0e-12f);
         }
-        act[a] = active;
+        step[a] = 0;
+        y[a] = 1.0f;
         if (!active) {
             // Constant coordinate: a ray outside the slab by more than E has no
             // in-range segment at all.
             if (sv[a] < -0.5f - E || sv[a] > (float)g.n[a] - 0.5f + E) return;
-            dirq[a] = 1.0f;
-            step[a] = 0;
             continue;
         }
-        dirq[a] = (MODE == MODE_OTF) ? 1.0f / dir : dir;
-        const bool up = dir > 0.0f;
+        y[a] = 1.0f / dir[a];
+        const bool up = dir[a] > 0.0f;
         step[a] = up ? 1 : -1;
-        const float t_in = crossing<MODE>(up ? 0 : g.n[a], sv[a], dirq[a]);
-        const float t_out = crossing<MODE>(up ? g.n[a] : 0, sv[a], dirq[a]);
-        const float tau = 4.0f * E / fabsf(dir) + 9.5367431640625e-07f;  // + 2^-20
+        const float t_in = crossing_p<MODE>(up ? 0 : g.n[a], sv[a], dir[a], y[a]);
+        const float t_out = crossing_p<MODE>(up ? g.n[a] : 0, sv[a], dir[a], y[a]);
+        const float tau = 4.0f * E / fabsf(dir[a]) + 9.5367431640625e-07f;  // + 2^-20
         t_lo = fmaxf(t_lo, t_in - tau);
         t_hi = fminf(t_hi, t_out + tau);
     }
     if (!(t_lo <= t_hi)) return;
 
-    // Jump each axis to its first crossing >= t_lo (crossings are monotone).
+    // Jump each axis to its first crossing >= t_lo.
+    Merge<ND, MODE> mg;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        next[a] = kInf;
-        pl[a] = 0;
-        if (!act[a]) continue;
-        const int n = g.n[a];
-        const float c = sv[a] + t_lo * (dv[a] - sv[a]);
-        int p;
-        // The clamps are written so that a NaN guess lands inside [-1, n+1]:
-        // both search loops are then bounded by the grid size.
-        if (step[a] > 0) {
-            float f = ceilf(c + 0.5f);
-            p = (f >= 0.0f) ? ((f <= (float)(n + 1)) ? (int)f : n + 1) : 0;
-            while (p > 0 && crossing<MODE>(p - 1, sv[a], dirq[a]) >= t_lo) --p;
-            while (p <= n && crossing<MODE>(p, sv[a], dirq[a]) < t_lo) ++p;
-        } else {
-            float f = floorf(c + 0.5f);
-            p = (f >= -1.0f) ? ((f <= (float)n) ? (int)f : n) : -1;
-            while (p < n && crossing<MODE>(p + 1, sv[a], dirq[a]) >= t_lo) ++p;
-            while (p >= 0 && crossing<MODE>(p, sv[a], dirq[a]) < t_lo) --p;
+        if (step[a] == 0) continue;
+        const int p = first_plane<MODE>(sv[a], dir[a], y[a], g.n[a], step[a], t_lo);
+        float t = kInf;
+        if (p >= 0 && p <= g.n[a]) {
+            t = crossing_p<MODE>(p, sv[a], dir[a], y[a]);
+            if (t > 1.0f) t = kInf;
         }
-        pl[a] = p;
-        if (p >= 0 && p <= n) {
-            float t = crossing<MODE>(p, sv[a], dirq[a]);
-            next[a] = (t > 1.0f) ? kInf : t;
-        }
+        mg.set(a, sv[a], dir[a], y[a], p, step[a], g.n[a], t);
     }
-
-    auto pop = [&](float& t) -> bool {
-        float m = next[0];
-#pragma unroll
-        for (int a = 1; a < ND; ++a) m = fminf(m, next[a]);
-        if (!(m <= t_hi)) return false;
-        int ax = ND - 1;
-#pragma unroll
-        for (int a = ND - 2; a >= 0; --a)
-            if (next[a] == m) ax = a;
-#pragma unroll
-        for (int a = 0; a < ND; ++a) {
-            if (a == ax) {
-                int p = pl[a] + step[a];
-                pl[a] = p;
-                float tn = kInf;
-                if (p >= 0 && p <= g.n[a]) {
-                    tn = crossing<MODE>(p, sv[a], dirq[a]);
-                    if (tn > 1.0f) tn = kInf;
-                }
-                next[a] = tn;
-            }
-        }
-        t = m;
-        return true;
-    };
 
     float t0;
     if (MODE == MODE_OTF && t_lo == 0.0f) {
         t0 = 0.0f;  // endpoint inserted by collect_hits (3d.cu:890)
-    } else if (!pop(t0)) {
+    } else if (!mg.pop(t_hi, t0)) {
         return;
     }
     const bool close_at_one = (MODE == MODE_OTF) && (t_hi == 1.0f);
 
+    MidIndex<ND, DIAG> mi;
+    mi.init(g);
     float p0[ND];
 #pragma unroll
     for (int a = 0; a < ND; ++a) p0[a] = src[a] + t0 * vw[a];
@@ -276,7 +442,7 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
     while (true) {
         float t1;
         bool last = false;
-        if (!pop(t1)) {
+        if (!mg.pop(t_hi, t1)) {
             if (close_at_one && t0 < 1.0f) {
                 t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
                 last = true;
@@ -287,7 +453,7 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
         float c[ND];
         float len;
         if (MODE == MODE_TORCH) {
-            float p1[ND], m[ND];
+            float p1[ND];
             float ss = 0.0f;
 #pragma unroll
             for (int a = 0; a < ND; ++a) {
@@ -295,11 +461,11 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
                 const float dd = p1[a] - p0[a];
                 const float sq = dd * dd;
                 ss = (a == 0) ? sq : ss + sq;
-                m[a] = 0.5f * (p0[a] + p1[a]);
-                p0[a] = p1[a];
             }
-            len = __builtin_sqrtf(ss);
-            mid_coords<ND, MODE, DIAG>(g, m, c);
+            len = seg_sqrt(ss);
+            mi.from_ends(p0, p1, c);
+#pragma unroll
+            for (int a = 0; a < ND; ++a) p0[a] = p1[a];
         } else {
             float m[ND];
             const float tm = 0.5f * (t0 + t1);
@@ -309,10 +475,9 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
             mid_coords<ND, MODE, DIAG>(g, m, c);
         }
         t0 = t1;
-        if (len > 0.0f) {
-            const int idx = flat_voxel<ND>(g, c);
-            if (idx >= 0) visit(idx, len);
-        }
+        uint32_t idx;
+        const bool in = flat_voxel_bounded<ND>(g, c, &idx);
+        if (in && len > 0.0f) visit(idx, len);
         if (last) break;
     }
 }
@@ -325,65 +490,29 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
 template <int ND, class Emit>
 __host__ __device__ __forceinline__ int emit_crossings(const Grid<ND>& g, const float* src, const float* dst,
                                                        Emit&& emit) {
-    float sv[ND], dv[ND], dir[ND], next[ND];
-    int pl[ND], step[ND];
-    const float kInf = __builtin_huge_valf();
+    float sv[ND], dv[ND];
     to_voxel<ND, MODE_TORCH>(g, src, sv);
     to_voxel<ND, MODE_TORCH>(g, dst, dv);
+    Merge<ND, MODE_TORCH> mg;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        dir[a] = dv[a] - sv[a];
-        next[a] = kInf;
-        pl[a] = 0;
-        step[a] = 0;
-        const bool active = (g.zero_rule == ZERO_EXACT) ? (dir[a] != 0.0f) : !(fabsf(dir[a]) < 1.0e-12f);
+        const float dir = dv[a] - sv[a];
+        const bool active = (g.zero_rule == ZERO_EXACT) ? (dir != 0.0f) : !(fabsf(dir) < 1.0e-12f);
         if (!active) continue;
-        const int n = g.n[a];
-        int p;
-        // First crossing with t >= 0 in traversal order (monotone in p).
-        if (dir[a] > 0.0f) {
-            step[a] = 1;
-            const float f = ceilf(sv[a] + 0.5f);
-            p = (f >= 0.0f) ? ((f <= (float)(n + 1)) ? (int)f : n + 1) : 0;
-            while (p > 0 && crossing<MODE_TORCH>(p - 1, sv[a], dir[a]) >= 0.0f) --p;
-            while (p <= n && crossing<MODE_TORCH>(p, sv[a], dir[a]) < 0.0f) ++p;
-        } else {
-            step[a] = -1;
-            const float f = floorf(sv[a] + 0.5f);
-            p = (f >= -1.0f) ? ((f <= (float)n) ? (int)f : n) : -1;
-            while (p < n && crossing<MODE_TORCH>(p + 1, sv[a], dir[a]) >= 0.0f) ++p;
-            while (p >= 0 && crossing<MODE_TORCH>(p, sv[a], dir[a]) < 0.0f) --p;
+        const float y = 1.0f / dir;
+        const int step = dir > 0.0f ? 1 : -1;
+        const int p = first_plane<MODE_TORCH>(sv[a], dir, y, g.n[a], step, 0.0f);
+        float t = __builtin_huge_valf();
+        if (p >= 0 && p <= g.n[a]) {
+            t = crossing_p<MODE_TORCH>(p, sv[a], dir, y);
+            if (t > 1.0f) t = __builtin_huge_valf();
         }
-        pl[a] = p;
-        if (p >= 0 && p <= n) {
-            const float t = crossing<MODE_TORCH>(p, sv[a], dir[a]);
-            next[a] = (t > 1.0f) ? kInf : t;
-        }
+        mg.set(a, sv[a], dir, y, p, step, g.n[a], t);
     }
     int count = 0;
-    while (true) {
-        float m = next[0];
-#pragma unroll
-        for (int a = 1; a < ND; ++a) m = fminf(m, next[a]);
-        if (!(m <= 1.0f)) break;
-        int ax = ND - 1;
-#pragma unroll
-        for (int a = ND - 2; a >= 0; --a)
-            if (next[a] == m) ax = a;
-#pragma unroll
-        for (int a = 0; a < ND; ++a) {
-            if (a == ax) {
-                const int p = pl[a] + step[a];
-                pl[a] = p;
-                float tn = kInf;
-                if (p >= 0 && p <= g.n[a]) {
-                    tn = crossing<MODE_TORCH>(p, sv[a], dir[a]);
-                    if (tn > 1.0f) tn = kInf;
-                }
-                next[a] = tn;
-            }
-        }
-        emit(m);
+    float t;
+    while (mg.pop(1.0f, t)) {
+        emit(t);
         ++count;
     }
     return count;
@@ -392,12 +521,13 @@ __host__ __device__ __forceinline__ int emit_crossings(const Grid<ND>& g, const 
 // Segment geometry of one pair of crossings given as t values, in the op order
 // of the torch consumer (3d_torch.py:219-271): world points src + t (dst-src),
 // length |P1 - P0|, midpoint voxel by round-half-even.  p0 holds P(t0) on
-// entry and P(t1) on return.  Returns the flat voxel or -1; *len receives the
-// length.
+// entry and P(t1) on return.  Returns whether the voxel is in the grid; *idx
+// and *len receive the flat voxel index and the length.
 template <int ND, bool DIAG>
-__host__ __device__ __forceinline__ int segment_from_t(const Grid<ND>& g, const float* src, const float* vw,
-                                                       float* p0, float t1, float* len) {
-    float p1[ND], m[ND], c[ND];
+__host__ __device__ __forceinline__ bool segment_from_t(const Grid<ND>& g, const MidIndex<ND, DIAG>& mi,
+                                                        const float* src, const float* vw, float* p0, float t1,
+                                                        uint32_t* idx, float* len) {
+    float p1[ND], c[ND];
     float ss = 0.0f;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
@@ -405,12 +535,12 @@ __host__ __device__ __forceinline__ int segment_from_t(const Grid<ND>& g, const 
         const float dd = p1[a] - p0[a];
         const float sq = dd * dd;
         ss = (a == 0) ? sq : ss + sq;
-        m[a] = 0.5f * (p0[a] + p1[a]);
-        p0[a] = p1[a];
     }
-    *len = __builtin_sqrtf(ss);
-    mid_coords<ND, MODE_TORCH, DIAG>(g, m, c);
-    return flat_voxel<ND>(g, c);
+    *len = seg_sqrt(ss);
+    mi.from_ends(p0, p1, c);
+#pragma unroll
+    for (int a = 0; a < ND; ++a) p0[a] = p1[a];
+    return flat_voxel<ND>(g, c, idx);
 }
 
 }  // namespace ctlab
