@@ -102,6 +102,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-views", type=int, default=8)
     ap.add_argument("--plain-order", action="store_true", help="one ray per lane in storage order (no tiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the full-size adjoint check")
+    ap.add_argument("--back", choices=["brick", "atomic"], default="brick",
+                    help="back projector: LDS bricks (no global atomics) or per-ray atomics")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projector shards over rays "
@@ -127,6 +129,18 @@ def main(argv=None) -> int:
     print(f"[bench] n_ray={n_ray} segments={S}", file=sys.stderr, flush=True)
 
     det = None if a.plain_order else (CFG["det_ny"], CFG["det_nx"])
+    frames = None
+    if a.back == "brick":
+        frames = _C.cone_frames(src, dst, (CFG["det_ny"], CFG["det_nx"]), minv, b, (n, n, n))
+        n_valid = int((frames[:, 15] == 1).sum().item())
+        print(f"[bench] detector frames: {n_valid}/{frames.shape[0]} views fit the flat-panel model",
+              file=sys.stderr, flush=True)
+
+    def back(y):
+        if a.back == "brick":
+            return _C.back_project_3d_brick(y, src, dst, (CFG["det_ny"], CFG["det_nx"]), frames, minv, b, n, n, n,
+                                            mode=mode)
+        return _C.back_project_3d_on_the_fly_cuda(y, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
 
     def step(ev=None):
         if ev is not None:
@@ -134,7 +148,7 @@ def main(argv=None) -> int:
         sino = _C.forward_project_3d_on_the_fly_cuda(vol, src, dst, minv, b, mode=mode, det_shape=det)
         if ev is not None:
             ev[1].record()
-        bk = _C.back_project_3d_on_the_fly_cuda(sino, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
+        bk = back(sino)
         if ev is not None:
             ev[2].record()
         return sino, bk
@@ -159,7 +173,7 @@ def main(argv=None) -> int:
     if not a.no_adjoint:
         # Config check: <A x, y> vs <x, A^T y> with y ~ N(0, 1) (seed 2), FP64 dot products.
         y = torch.from_numpy(np.random.default_rng(2).standard_normal(n_ray, dtype=np.float32)).to(dev)
-        aty = _C.back_project_3d_on_the_fly_cuda(y, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
+        aty = back(y)
         lhs = float(torch.dot(sino[0].double(), y.double()))
         rhs = float(torch.dot(vol.reshape(-1).double(), aty.reshape(-1).double()))
         adjoint = abs(lhs - rhs) / abs(lhs)
@@ -191,6 +205,7 @@ def main(argv=None) -> int:
             "segments": S,
             "semantics": a.mode,
             "schedule": "plain" if a.plain_order else "detector-tiled (64 rows x 4 cols per block)",
+            "back_projector": a.back,
             "adjoint_rel_err": adjoint,
             "fwd_ms": fwd_ms,
             "back_ms": back_ms,

@@ -353,6 +353,51 @@ def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, 
     return out
 
 
+# ---------------------------------------------------------------------------
+# Atomic-free back projection of flat-panel view stacks (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+FRAME_FLOATS = 16
+
+
+def cone_frames(src, dst, det_shape, M_inv, b, shape):
+    """float32 [n_view, 16] detector frames of rays ordered [view][rows][cols]."""
+    name = "cone_frames"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 3)
+    rows, cols = (int(x) for x in det_shape)
+    _require(rows > 0 and cols > 0 and n_ray % (rows * cols) == 0, f"{name}: n_ray is not a multiple of rows*cols")
+    n_view = n_ray // (rows * cols)
+    frames = torch.empty((n_view, FRAME_FLOATS), device=src.device, dtype=torch.float32)
+    if n_view:
+        with torch.cuda.device(src.device):
+            N.call("ctlab_cone_frames_3d", N.ptr(src), N.ptr(dst), n_view, rows, cols, N.ptr(M_inv), N.ptr(b),
+                   *[int(x) for x in shape], N.ptr(frames), N.stream_of(src))
+    return frames
+
+
+def back_project_3d_brick(sino, src, dst, det_shape, frames, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH,
+                          out=None, accumulate: bool = False):
+    """[B, n_x, n_y, n_z] back projection by LDS-resident bricks (same segments as
+    back_project_3d_on_the_fly_cuda, no global atomics)."""
+    name = "back_project_3d_brick"
+    _check_f32(name, sino, src, dst, frames, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    rows, cols = (int(x) for x in det_shape)
+    _require(n_ray % (rows * cols) == 0, f"{name}: n_ray is not a multiple of rows*cols")
+    n_view = n_ray // (rows * cols)
+    _require(frames.shape == (n_view, FRAME_FLOATS), f"{name}: frames must be [n_view, {FRAME_FLOATS}]")
+    if out is None:
+        out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+        accumulate = False
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_brick", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
+               N.ptr(frames), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
+               N.ptr(out), N.stream_of(sino))
+    return out
+
+
 REFERENCE_NAMES = (
     "compute_intersections_2d", "forward_project_2d_cuda", "back_project_2d_cuda",
     "forward_project_2d_on_the_fly_cuda", "back_project_2d_on_the_fly_cuda",

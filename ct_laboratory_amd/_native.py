@@ -51,6 +51,9 @@ _SIGS = {
     "ctlab_forward_project_3d_u16": [_P, _I64, _I64, _I64, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _P, _P],
     "ctlab_back_project_3d_u16": [_P, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P,
                                   _P],
+    "ctlab_cone_frames_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _I64, _I64, _I64, _P, _P],
+    "ctlab_back_project_3d_brick": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
+                                    _P, _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }

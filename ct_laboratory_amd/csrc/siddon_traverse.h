@@ -70,6 +70,27 @@ __host__ __device__ __forceinline__ Grid<ND> load_grid(const int* n, const float
     return g;
 }
 
+// Voxel window [lo, lo+ext) per axis.  Segments are produced only when their
+// rounded voxel lies in the window, and the visitor receives the flat index
+// within the window ((i*ext_y + j)*ext_z + k).  The full grid is lo = 0,
+// ext = n, where the window index is the reference's flat index.
+template <int ND>
+struct Window {
+    int lo[ND];
+    int ext[ND];
+};
+
+template <int ND>
+__host__ __device__ __forceinline__ Window<ND> full_window(const Grid<ND>& g) {
+    Window<ND> w;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        w.lo[a] = 0;
+        w.ext[a] = g.n[a];
+    }
+    return w;
+}
+
 // Reference voxel-space transform of an endpoint: (p - b) @ M_inv^T with
 // separate rounding (3d_torch.py:100-103), or nvcc's contraction of
 // apply_affine_inverse_3d (3d.cu:21-36) in otf mode.
@@ -168,20 +189,20 @@ __host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const flo
 // traversal only evaluates midpoints within a few voxels of the grid (and the
 // grid dimensions are below 2^24 with E < 1), so |c| stays far below 2^22.
 template <int ND>
-__host__ __device__ __forceinline__ bool flat_voxel_bounded(const Grid<ND>& g, const float* c, uint32_t* idx) {
+__host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w, const float* c, uint32_t* idx) {
     const float kMagic = 12582912.0f;  // 1.5 * 2^23
     uint32_t r[ND];
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const float f = c[a] + kMagic;
-        r[a] = (uint32_t)(__builtin_bit_cast(int32_t, f) - 0x4b400000);
-        ok = ok && (r[a] < (uint32_t)g.n[a]);
+        r[a] = (uint32_t)(__builtin_bit_cast(int32_t, f) - 0x4b400000 - w.lo[a]);
+        ok = ok && (r[a] < (uint32_t)w.ext[a]);
     }
     if (ND == 3) {
-        *idx = mad24(r[0], (uint32_t)(g.n[1] * g.n[2]), mad24(r[1], (uint32_t)g.n[2], r[ND - 1]));
+        *idx = mad24(r[0], (uint32_t)(w.ext[1] * w.ext[2]), mad24(r[1], (uint32_t)w.ext[2], r[ND - 1]));
     } else {
-        *idx = mad24(r[0], (uint32_t)g.n[ND - 1], r[ND - 1]);
+        *idx = mad24(r[0], (uint32_t)w.ext[ND - 1], r[ND - 1]);
     }
     return ok;
 }
@@ -220,24 +241,26 @@ __host__ __device__ __forceinline__ float crossing_p(int p, float s, float d, fl
     return crossing_f<MODE>((float)p - 0.5f, s, d, y);
 }
 
-// First plane index p (in the axis' traversal direction) whose crossing is
-// >= t_lo, or one past the grid if none.  Crossings are monotone in p, so the
-// guess from the coordinate at t_lo is corrected by at most a few steps; the
-// clamps keep a NaN guess inside [-1, n+1] so both loops stay bounded.
+// First plane index p in [plo, phi] (in the axis' traversal direction) whose
+// crossing is >= t_lo, or one past the range if none.  Crossings are monotone
+// in p, so the guess from the coordinate at t_lo is corrected by at most a few
+// steps; the clamps keep a NaN guess inside [plo-1, phi+1] so both loops stay
+// bounded.
 template <int MODE>
-__host__ __device__ __forceinline__ int first_plane(float s, float d, float y, int n, int step, float t_lo) {
+__host__ __device__ __forceinline__ int first_plane(float s, float d, float y, int plo, int phi, int step,
+                                                    float t_lo) {
     const float c = s + t_lo * d;
     int p;
     if (step > 0) {
         const float f = ceilf(c + 0.5f);
-        p = (f >= 0.0f) ? ((f <= (float)(n + 1)) ? (int)f : n + 1) : 0;
-        while (p > 0 && crossing_p<MODE>(p - 1, s, d, y) >= t_lo) --p;
-        while (p <= n && crossing_p<MODE>(p, s, d, y) < t_lo) ++p;
+        p = (f >= (float)plo) ? ((f <= (float)(phi + 1)) ? (int)f : phi + 1) : plo;
+        while (p > plo && crossing_p<MODE>(p - 1, s, d, y) >= t_lo) --p;
+        while (p <= phi && crossing_p<MODE>(p, s, d, y) < t_lo) ++p;
     } else {
         const float f = floorf(c + 0.5f);
-        p = (f >= -1.0f) ? ((f <= (float)n) ? (int)f : n) : -1;
-        while (p < n && crossing_p<MODE>(p + 1, s, d, y) >= t_lo) ++p;
-        while (p >= 0 && crossing_p<MODE>(p, s, d, y) < t_lo) --p;
+        p = (f >= (float)(plo - 1)) ? ((f <= (float)phi) ? (int)f : phi) : plo - 1;
+        while (p < phi && crossing_p<MODE>(p + 1, s, d, y) >= t_lo) ++p;
+        while (p >= plo && crossing_p<MODE>(p, s, d, y) < t_lo) --p;
     }
     return p;
 }
@@ -254,10 +277,10 @@ struct Merge {
     float n0 = __builtin_huge_valf(), n1 = __builtin_huge_valf(), n2 = __builtin_huge_valf();
 
     // Initialises axis a (a is a compile-time constant after unrolling).
-    __host__ __device__ __forceinline__ void set(int a, float s, float d, float y, int p, int step, int n,
-                                                 float t_next) {
+    __host__ __device__ __forceinline__ void set(int a, float s, float d, float y, int p, int step, int plo,
+                                                 int phi, float t_next) {
         const float f = (float)p - 0.5f;
-        const float e = (step > 0) ? (float)n + 0.5f : -1.5f;
+        const float e = (step > 0) ? (float)phi + 0.5f : (float)plo - 1.5f;
         const float st = (float)step;
         if (a == 0) { s0 = s; d0 = d; y0 = y; f0 = f; st0 = st; e0 = e; n0 = t_next; }
         if (a == 1) { s1 = s; d1 = d; y1 = y; f1 = f; st1 = st; e1 = e; n1 = t_next; }
@@ -339,12 +362,15 @@ struct MidIndex {
     }
 };
 
-// Visits every segment of one ray whose rounded midpoint lies in the grid,
-// in increasing t, as visit(flat_index, length).  Zero-length segments are
-// not visited (they contribute exactly zero to both projections).
+// Visits every segment of one ray whose rounded midpoint lies in the window,
+// in increasing t, as visit(window_flat_index, length).  Zero-length segments
+// are not visited (they contribute exactly zero to both projections).  The
+// window's bounding planes take the place of the grid's in the clipping
+// argument above, so each segment of the full traversal is visited by exactly
+// the window that contains its voxel.
 template <int ND, int MODE, bool DIAG, class Visit>
-__host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const float* src, const float* dst,
-                                                   Visit&& visit) {
+__host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, const Window<ND>& win,
+                                                          const float* src, const float* dst, Visit&& visit) {
     if (!g.finite) return;  // the reference's indices are all out of range then
     // Non-finite endpoints make every reference world point INF or NaN, and
     // every such segment is masked (3d_torch.py:233-260): no contribution.
@@ -397,14 +423,15 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
         if (!active) {
             // Constant coordinate: a ray outside the slab by more than E has no
             // in-range segment at all.
-            if (sv[a] < -0.5f - E || sv[a] > (float)g.n[a] - 0.5f + E) return;
+            if (sv[a] < (float)win.lo[a] - 0.5f - E || sv[a] > (float)(win.lo[a] + win.ext[a]) - 0.5f + E) return;
             continue;
         }
         y[a] = 1.0f / dir[a];
         const bool up = dir[a] > 0.0f;
         step[a] = up ? 1 : -1;
-        const float t_in = crossing_p<MODE>(up ? 0 : g.n[a], sv[a], dir[a], y[a]);
-        const float t_out = crossing_p<MODE>(up ? g.n[a] : 0, sv[a], dir[a], y[a]);
+        const int plo = win.lo[a], phi = win.lo[a] + win.ext[a];
+        const float t_in = crossing_p<MODE>(up ? plo : phi, sv[a], dir[a], y[a]);
+        const float t_out = crossing_p<MODE>(up ? phi : plo, sv[a], dir[a], y[a]);
         const float tau = 4.0f * E / fabsf(dir[a]) + 9.5367431640625e-07f;  // + 2^-20
         t_lo = fmaxf(t_lo, t_in - tau);
         t_hi = fminf(t_hi, t_out + tau);
@@ -416,13 +443,14 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         if (step[a] == 0) continue;
-        const int p = first_plane<MODE>(sv[a], dir[a], y[a], g.n[a], step[a], t_lo);
+        const int plo = win.lo[a], phi = win.lo[a] + win.ext[a];
+        const int p = first_plane<MODE>(sv[a], dir[a], y[a], plo, phi, step[a], t_lo);
         float t = kInf;
-        if (p >= 0 && p <= g.n[a]) {
+        if (p >= plo && p <= phi) {
             t = crossing_p<MODE>(p, sv[a], dir[a], y[a]);
             if (t > 1.0f) t = kInf;
         }
-        mg.set(a, sv[a], dir[a], y[a], p, step[a], g.n[a], t);
+        mg.set(a, sv[a], dir[a], y[a], p, step[a], plo, phi, t);
     }
 
     float t0;
@@ -476,10 +504,17 @@ __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const floa
         }
         t0 = t1;
         uint32_t idx;
-        const bool in = flat_voxel_bounded<ND>(g, c, &idx);
+        const bool in = flat_voxel_bounded<ND>(win, c, &idx);
         if (in && len > 0.0f) visit(idx, len);
         if (last) break;
     }
+}
+
+// The whole grid: visit(flat_index, length) with the reference's flat index.
+template <int ND, int MODE, bool DIAG, class Visit>
+__host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const float* src, const float* dst,
+                                                   Visit&& visit) {
+    trace_ray_window<ND, MODE, DIAG>(g, full_window(g), src, dst, visit);
 }
 
 // Emits, in increasing order, every plane crossing t in [0, 1] of the whole
@@ -501,13 +536,13 @@ __host__ __device__ __forceinline__ int emit_crossings(const Grid<ND>& g, const 
         if (!active) continue;
         const float y = 1.0f / dir;
         const int step = dir > 0.0f ? 1 : -1;
-        const int p = first_plane<MODE_TORCH>(sv[a], dir, y, g.n[a], step, 0.0f);
+        const int p = first_plane<MODE_TORCH>(sv[a], dir, y, 0, g.n[a], step, 0.0f);
         float t = __builtin_huge_valf();
         if (p >= 0 && p <= g.n[a]) {
             t = crossing_p<MODE_TORCH>(p, sv[a], dir, y);
             if (t > 1.0f) t = __builtin_huge_valf();
         }
-        mg.set(a, sv[a], dir, y, p, step, g.n[a], t);
+        mg.set(a, sv[a], dir, y, p, step, 0, g.n[a], t);
     }
     int count = 0;
     float t;

@@ -177,6 +177,33 @@ int ctlab_back_project_3d_u16(const float* sino, int64_t batch, const uint16_t* 
                               const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
                               int accumulate, float* out, ctlab_stream_t stream);
 
+/* ---------------------------------------------------------------------- */
+/* Atomic-free back projection for flat-panel view stacks (no reference    */
+/* counterpart; an alternative schedule of back_project_3d_on_the_fly with  */
+/* identical segments)                                                      */
+/* ---------------------------------------------------------------------- */
+
+#define CTLAB_FRAME_FLOATS 16
+
+/* Per-view detector frames for rays ordered [n_view][det_rows][det_cols]:
+ * frames [n_view, 16] floats (source, pixel (0,0), row and column steps in
+ * voxel coordinates, error bound, fit deviations, valid flag).  A view is
+ * valid when its rays share one source and its pixels lie on an affine grid
+ * within a quarter pixel.  Computed once per geometry. */
+int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows, int64_t det_cols,
+                         const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, float* frames,
+                         ctlab_stream_t stream);
+
+/* Same result as ctlab_back_project_3d_otf (every segment, the same voxel and
+ * length), computed by 32^3 bricks accumulated in LDS over all views and
+ * written once: no global atomics.  Invalid views are traced over their whole
+ * detector (slower, still exact).  out [batch, n_x, n_y, n_z] is overwritten
+ * (or added to when accumulate != 0). */
+int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
+                                int64_t n_view, int64_t det_rows, int64_t det_cols, const float* frames,
+                                const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
+                                int accumulate, float* out, ctlab_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

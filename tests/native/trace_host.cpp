@@ -41,3 +41,58 @@ extern "C" int64_t host_trace(int ndim, int mode, int zero_rule, const int* n, c
     if (ndim == 2) return dispatch<2>(mode, zero_rule, n, minv, b, src, dst, n_ray, cap, idx, len, counts);
     return dispatch<3>(mode, zero_rule, n, minv, b, src, dst, n_ray, cap, idx, len, counts);
 }
+
+// Window variant: segments whose voxel lies in [lo, lo+ext), reported with
+// their full-grid flat index.
+template <int ND, int MODE, bool DIAG>
+static void run_window(const Grid<ND>& g, const Window<ND>& w, const float* src, const float* dst, int64_t n_ray,
+                       int64_t cap, int64_t* idx, float* len, int64_t* counts) {
+    for (int64_t r = 0; r < n_ray; ++r) {
+        int64_t k = 0;
+        int64_t* io = idx + r * cap;
+        float* lo = len + r * cap;
+        trace_ray_window<ND, MODE, DIAG>(g, w, src + r * ND, dst + r * ND, [&](uint32_t li, float l) {
+            // window-local flat index -> grid flat index
+            int64_t rem = li, gi = 0;
+            int64_t loc[ND];
+            for (int a = ND - 1; a >= 0; --a) {
+                loc[a] = rem % w.ext[a];
+                rem /= w.ext[a];
+            }
+            for (int a = 0; a < ND; ++a) gi = gi * g.n[a] + (loc[a] + w.lo[a]);
+            if (k < cap) { io[k] = gi; lo[k] = l; }
+            ++k;
+        });
+        counts[r] = k;
+    }
+}
+
+extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const int* n, const float* minv,
+                                     const float* b, const int* lo, const int* ext, const float* src,
+                                     const float* dst, int64_t n_ray, int64_t cap, int64_t* idx, float* len,
+                                     int64_t* counts) {
+    if (ndim == 3) {
+        Grid<3> g = load_grid<3>(n, minv, b, zero_rule);
+        Window<3> w;
+        for (int a = 0; a < 3; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
+        if (mode == MODE_TORCH) {
+            if (g.diag) run_window<3, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else run_window<3, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
+        } else {
+            if (g.diag) run_window<3, MODE_OTF, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else run_window<3, MODE_OTF, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
+        }
+    } else {
+        Grid<2> g = load_grid<2>(n, minv, b, zero_rule);
+        Window<2> w;
+        for (int a = 0; a < 2; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
+        if (mode == MODE_TORCH) {
+            if (g.diag) run_window<2, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else run_window<2, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
+        } else {
+            if (g.diag) run_window<2, MODE_OTF, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else run_window<2, MODE_OTF, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
+        }
+    }
+    return 0;
+}

@@ -77,3 +77,63 @@ def test_conebeam_rays(host, mode):
     src, dst = build_conebeam_3d_geometry(10, 40, 24, 1000.0, 1500.0, det_spacing=1.5 * 256 / 40)
     b = np.full(3, -(256 - 1) / 2, np.float32)
     _check(host, (256, 256, 256), np.eye(3), b, src.numpy(), dst.numpy(), mode)
+
+
+@pytest.fixture(scope="module")
+def host_window():
+    L = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    L.host_trace_window.restype = ctypes.c_int64
+    L.host_trace_window.argtypes = [ctypes.c_int] * 3 + [P] * 7 + [ctypes.c_int64] * 2 + [P] * 3
+
+    def run(shape, minv, b, lo, ext, src, dst, mode, zr):
+        nd = len(shape)
+        arrs = [np.array(shape, np.int32), np.ascontiguousarray(minv, np.float32).reshape(-1),
+                np.ascontiguousarray(b, np.float32), np.array(lo, np.int32), np.array(ext, np.int32),
+                np.ascontiguousarray(src, np.float32), np.ascontiguousarray(dst, np.float32)]
+        N = arrs[5].shape[0]
+        cap = sum(shape) + nd + 2
+        idx = np.zeros((N, cap), np.int64)
+        ln = np.zeros((N, cap), np.float32)
+        cnt = np.zeros(N, np.int64)
+        p = lambda a: a.ctypes.data_as(P)  # noqa: E731
+        L.host_trace_window(nd, {"torch": 0, "otf": 1}[mode], {"thresh": 0, "exact": 1}[zr], *[p(a) for a in arrs],
+                            N, cap, p(idx), p(ln), p(cnt))
+        return [(idx[r, :cnt[r]], ln[r, :cnt[r]]) for r in range(N)]
+
+    return run
+
+
+def _partition(rng, n, parts):
+    cuts = np.sort(rng.choice(np.arange(1, n), size=min(parts - 1, n - 1), replace=False)) if n > 1 else []
+    edges = [0] + list(cuts) + [n]
+    return [(edges[i], edges[i + 1] - edges[i]) for i in range(len(edges) - 1)]
+
+
+@pytest.mark.parametrize("mode", ["torch", "otf"])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_windows_partition_the_traversal(host, host_window, mode, nd):
+    """Splitting the grid into windows visits every segment exactly once: the
+    per-ray union over the windows equals the full traversal."""
+    import itertools
+
+    rng = np.random.default_rng(77 + nd)
+    for trial in range(6):
+        shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial == 5))
+        kind = ["far", "inside", "lattice"][trial % 3]
+        src, dst = random_rays(rng, shape, s, 600, kind)
+        zr = "exact" if (nd == 2 and mode == "torch") else "thresh"
+        full = host(shape, minv, b, src, dst, mode, zr)
+        parts = [_partition(rng, shape[a], 3) for a in range(nd)]
+        got = [([], []) for _ in range(len(src))]
+        for combo in itertools.product(*parts):
+            lo = [c[0] for c in combo]
+            ext = [c[1] for c in combo]
+            res = host_window(shape, minv, b, lo, ext, src, dst, mode, zr)
+            for r in range(len(src)):
+                got[r][0].extend(res[r][0].tolist())
+                got[r][1].extend(res[r][1].tolist())
+        for r in range(len(src)):
+            a = sorted(zip(full[r][0].tolist(), full[r][1].tolist()))
+            bb = sorted(zip(got[r][0], got[r][1]))
+            assert a == bb, (trial, r)
