@@ -25,6 +25,10 @@ constexpr int kBrick = 32;                  // brick edge, voxels
 constexpr int kBrickThreads = 1024;         // 16 waves
 constexpr int kViewChunk = kBrickThreads;   // views whose shadows sit in LDS at once
 constexpr int kFrameFloats = 16;
+// LDS strides of the brick: padded so that 32 lanes stepping along any one
+// axis hit 32 different banks (33 and 32*33+1 are odd multiples mod 32).
+constexpr int kStrideZ = 1, kStrideY = kBrick + 1, kStrideX = kBrick * (kBrick + 1) + 1;
+constexpr int kBrickLds = kBrick * kStrideX;
 
 struct Dims3 {
     int n[3];
@@ -123,50 +127,55 @@ __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const in
     // Any ray with a segment whose rounded voxel is in the box passes within E
     // of it; the model line S -> O + rR + cC of its pixel deviates from the ray
     // by at most the source and pixel deviations.  Margin: one voxel more.
-    const double m = 1.0 + (double)f[F_E] + (double)f[F_DEVS] + (double)f[F_DEVD];
-    const double S[3] = {f[F_S], f[F_S + 1], f[F_S + 2]};
-    const double O[3] = {f[F_O], f[F_O + 1], f[F_O + 2]};
-    const double B[3] = {-f[F_R], -f[F_R + 1], -f[F_R + 2]};
-    const double Cm[3] = {-f[F_C], -f[F_C + 1], -f[F_C + 2]};
-    const double rhs[3] = {O[0] - S[0], O[1] - S[1], O[2] - S[2]};
-    auto cross = [](const double* a, const double* c, double* o) {
+    // Float precision is ample: coordinates of ~1e3 voxels err by ~1e-4 voxels,
+    // far inside the margin and the one-pixel pad.
+    const float m = 1.0f + f[F_E] + f[F_DEVS] + f[F_DEVD];
+    const float S[3] = {f[F_S], f[F_S + 1], f[F_S + 2]};
+    const float B[3] = {-f[F_R], -f[F_R + 1], -f[F_R + 2]};
+    const float Cm[3] = {-f[F_C], -f[F_C + 1], -f[F_C + 2]};
+    const float rhs[3] = {f[F_O] - S[0], f[F_O + 1] - S[1], f[F_O + 2] - S[2]};
+    auto cross = [](const float* a, const float* c, float* o) {
         o[0] = a[1] * c[2] - a[2] * c[1];
         o[1] = a[2] * c[0] - a[0] * c[2];
         o[2] = a[0] * c[1] - a[1] * c[0];
     };
-    auto dot = [](const double* a, const double* c) { return a[0] * c[0] + a[1] * c[1] + a[2] * c[2]; };
-    double bxc[3], rxc[3], bxr[3];
+    auto dot = [](const float* a, const float* c) { return a[0] * c[0] + a[1] * c[1] + a[2] * c[2]; };
+    float bxc[3], rxc[3], bxr[3];
     cross(B, Cm, bxc);
     cross(rhs, Cm, rxc);
     cross(B, rhs, bxr);
-    double rmin = 1e300, rmax = -1e300, cmin = 1e300, cmax = -1e300;
+    const float num_l = dot(rhs, bxc);
+    float rmin = 3e38f, rmax = -3e38f, cmin = 3e38f, cmax = -3e38f;
     for (int k = 0; k < 8; ++k) {
-        double A[3];
+        float A[3];
         for (int a = 0; a < 3; ++a) {
-            const double x = (k >> a) & 1 ? (double)(lo[a] + ext[a]) - 0.5 + m : (double)lo[a] - 0.5 - m;
+            const float x = ((k >> a) & 1) ? (float)(lo[a] + ext[a]) - 0.5f + m : (float)lo[a] - 0.5f - m;
             A[a] = x - S[a];
         }
-        const double det = dot(A, bxc);
-        if (!(fabs(det) > 1e-30)) return false;
-        const double lam = dot(rhs, bxc) / det;
-        if (!(lam > 0.0)) return false;  // corner not in front of the source
-        const double r = dot(A, rxc) / det;
-        const double c = dot(A, bxr) / det;
-        rmin = fmin(rmin, r);
-        rmax = fmax(rmax, r);
-        cmin = fmin(cmin, c);
-        cmax = fmax(cmax, c);
+        const float det = dot(A, bxc);
+        if (!(fabsf(det) > 1e-20f)) return false;
+        if (!(num_l / det > 0.0f)) return false;  // corner not in front of the source
+        const float inv = 1.0f / det;
+        const float r = dot(A, rxc) * inv;
+        const float c = dot(A, bxr) * inv;
+        rmin = fminf(rmin, r);
+        rmax = fmaxf(rmax, r);
+        cmin = fminf(cmin, c);
+        cmax = fmaxf(cmax, c);
     }
-    const double pad = 2.0;
-    const double fr0 = floor(rmin - pad), fr1 = ceil(rmax + pad), fc0 = floor(cmin - pad), fc1 = ceil(cmax + pad);
-    *r0 = (int)fmax(fr0, 0.0);
-    *r1 = (int)fmin(fr1, (double)(rows - 1));
-    *c0 = (int)fmax(fc0, 0.0);
-    *c1 = (int)fmin(fc1, (double)(cols - 1));
-    if (!(fr1 >= 0.0) || !(fc1 >= 0.0) || !(fr0 <= (double)(rows - 1)) || !(fc0 <= (double)(cols - 1))) {
+    const float pad = 1.0f;
+    const float fr0 = floorf(rmin - pad), fr1 = ceilf(rmax + pad), fc0 = floorf(cmin - pad), fc1 = ceilf(cmax + pad);
+    if (!(fr1 >= 0.0f) || !(fc1 >= 0.0f) || !(fr0 <= (float)(rows - 1)) || !(fc0 <= (float)(cols - 1))) {
         *r0 = 0;
         *r1 = -1;  // empty
+        *c0 = 0;
+        *c1 = -1;
+        return true;
     }
+    *r0 = (int)fmaxf(fr0, 0.0f);
+    *r1 = (int)fminf(fr1, (float)(rows - 1));
+    *c0 = (int)fmaxf(fc0, 0.0f);
+    *c1 = (int)fminf(fc1, (float)(cols - 1));
     return true;
 }
 
@@ -176,8 +185,8 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
     int nbx, int nby, int nbz, int n_view, int rows, int cols, int view_chunk, const float* __restrict__ frames,
     const float* __restrict__ src, const float* __restrict__ dst, const float* __restrict__ minv,
     const float* __restrict__ b, int accumulate) {
-    __shared__ float acc[kBrick * kBrick * kBrick];
-    __shared__ int s_r0[kViewChunk], s_h[kViewChunk], s_c0[kViewChunk];
+    __shared__ float acc[kBrickLds];
+    __shared__ int s_r0[kViewChunk], s_w[kViewChunk], s_c0[kViewChunk];
     __shared__ uint32_t s_pre[kViewChunk + 1];
     __shared__ uint32_t s_wsum[kBrickThreads / 64];
 
@@ -189,8 +198,10 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
     win.lo[1] = by * kBrick;
     win.lo[2] = bz * kBrick;
     for (int a = 0; a < 3; ++a) win.ext[a] = min(kBrick, dims.n[a] - win.lo[a]);
-    const int nvox = win.ext[0] * win.ext[1] * win.ext[2];
-    for (int i = tid; i < nvox; i += kBrickThreads) acc[i] = 0.0f;
+    win.stride[0] = kStrideX;
+    win.stride[1] = kStrideY;
+    win.stride[2] = kStrideZ;
+    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0.0f;
 
     const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
     const int64_t bi = blockIdx.y;
@@ -211,7 +222,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
             }
             const int h = r1 - r0 + 1, w = c1 - c0 + 1;
             s_r0[tid] = r0;
-            s_h[tid] = h > 0 ? h : 0;
+            s_w[tid] = w > 0 ? w : 0;
             s_c0[tid] = c0;
             cnt = (h > 0 && w > 0) ? (uint32_t)h * (uint32_t)w : 0u;
         }
@@ -239,7 +250,8 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
         __syncthreads();
         const uint32_t total = s_pre[nv];
 
-        // 3. trace every (view, pixel) of the chunk's shadows, rows fastest
+        // 3. trace every (view, pixel) of the chunk's shadows, columns fastest
+        //    (neighbouring lanes read neighbouring src/dst/sino entries)
         for (uint64_t item = tid; item < total; item += kBrickThreads) {
             int lo = 0, hi = nv;  // largest view with s_pre[view] <= item
             while (hi - lo > 1) {
@@ -247,9 +259,9 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
                 if (s_pre[mid] <= item) lo = mid; else hi = mid;
             }
             const uint32_t local = (uint32_t)(item - s_pre[lo]);
-            const uint32_t h = (uint32_t)s_h[lo];
-            const int r = s_r0[lo] + (int)(local % h);
-            const int c = s_c0[lo] + (int)(local / h);
+            const uint32_t w = (uint32_t)s_w[lo];
+            const int r = s_r0[lo] + (int)(local / w);
+            const int c = s_c0[lo] + (int)(local % w);
             const int64_t ray = ((int64_t)(v0 + lo) * rows + r) * cols + c;
             const float val = sn[ray];
             if (val == 0.0f) continue;
@@ -267,13 +279,15 @@ __global__ __launch_bounds__(kBrickThreads) void k_back_brick(
         __syncthreads();
     }
 
-    // 4. write the brick
+    // 4. write the brick (k fastest: coalesced rows of the volume)
     float* __restrict__ o = out + bi * vol_elems;
     const int ey = win.ext[1], ez = win.ext[2];
+    const int nvox = win.ext[0] * ey * ez;
     for (int i = tid; i < nvox; i += kBrickThreads) {
         const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
         const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-        o[gi] = accumulate ? o[gi] + acc[i] : acc[i];
+        const float v = acc[ii * kStrideX + j * kStrideY + k];
+        o[gi] = accumulate ? o[gi] + v : v;
     }
 }
 

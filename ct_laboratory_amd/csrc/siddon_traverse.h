@@ -71,13 +71,15 @@ __host__ __device__ __forceinline__ Grid<ND> load_grid(const int* n, const float
 }
 
 // Voxel window [lo, lo+ext) per axis.  Segments are produced only when their
-// rounded voxel lies in the window, and the visitor receives the flat index
-// within the window ((i*ext_y + j)*ext_z + k).  The full grid is lo = 0,
-// ext = n, where the window index is the reference's flat index.
+// rounded voxel lies in the window, and the visitor receives the window index
+// sum_a (i_a - lo_a) * stride_a (the last axis has stride 1).  The full grid
+// is lo = 0, ext = n, stride = (n_y*n_z, n_z, 1): the reference's flat index.
+// Strides must stay below 2^24.
 template <int ND>
 struct Window {
     int lo[ND];
     int ext[ND];
+    int stride[ND];
 };
 
 template <int ND>
@@ -88,6 +90,9 @@ __host__ __device__ __forceinline__ Window<ND> full_window(const Grid<ND>& g) {
         w.lo[a] = 0;
         w.ext[a] = g.n[a];
     }
+    w.stride[ND - 1] = 1;
+#pragma unroll
+    for (int a = ND - 2; a >= 0; --a) w.stride[a] = w.stride[a + 1] * g.n[a + 1];
     return w;
 }
 
@@ -200,9 +205,9 @@ __host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w,
         ok = ok && (r[a] < (uint32_t)w.ext[a]);
     }
     if (ND == 3) {
-        *idx = mad24(r[0], (uint32_t)(w.ext[1] * w.ext[2]), mad24(r[1], (uint32_t)w.ext[2], r[ND - 1]));
+        *idx = mad24(r[0], (uint32_t)w.stride[0], mad24(r[1], (uint32_t)w.stride[1], r[ND - 1]));
     } else {
-        *idx = mad24(r[0], (uint32_t)w.ext[ND - 1], r[ND - 1]);
+        *idx = mad24(r[0], (uint32_t)w.stride[0], r[ND - 1]);
     }
     return ok;
 }

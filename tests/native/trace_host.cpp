@@ -52,12 +52,12 @@ static void run_window(const Grid<ND>& g, const Window<ND>& w, const float* src,
         int64_t* io = idx + r * cap;
         float* lo = len + r * cap;
         trace_ray_window<ND, MODE, DIAG>(g, w, src + r * ND, dst + r * ND, [&](uint32_t li, float l) {
-            // window-local flat index -> grid flat index
+            // window index (compact strides) -> grid flat index
             int64_t rem = li, gi = 0;
             int64_t loc[ND];
-            for (int a = ND - 1; a >= 0; --a) {
-                loc[a] = rem % w.ext[a];
-                rem /= w.ext[a];
+            for (int a = 0; a < ND; ++a) {
+                loc[a] = rem / w.stride[a];
+                rem %= w.stride[a];
             }
             for (int a = 0; a < ND; ++a) gi = gi * g.n[a] + (loc[a] + w.lo[a]);
             if (k < cap) { io[k] = gi; lo[k] = l; }
@@ -75,6 +75,7 @@ extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const in
         Grid<3> g = load_grid<3>(n, minv, b, zero_rule);
         Window<3> w;
         for (int a = 0; a < 3; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
+        w.stride[2] = 1; w.stride[1] = ext[2]; w.stride[0] = ext[1] * ext[2];
         if (mode == MODE_TORCH) {
             if (g.diag) run_window<3, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
             else run_window<3, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
@@ -86,6 +87,7 @@ extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const in
         Grid<2> g = load_grid<2>(n, minv, b, zero_rule);
         Window<2> w;
         for (int a = 0; a < 2; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
+        w.stride[1] = 1; w.stride[0] = ext[1];
         if (mode == MODE_TORCH) {
             if (g.diag) run_window<2, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
             else run_window<2, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
