@@ -376,24 +376,42 @@ def cone_frames(src, dst, det_shape, M_inv, b, shape):
     return frames
 
 
-def back_project_3d_brick(sino, src, dst, det_shape, frames, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH,
+def brick_plan(frames, det_shape, shape):
+    """int32 plan of the brick back projector for a geometry (see ctlab_brick_plan_3d)."""
+    name = "brick_plan"
+    _check_f32(name, frames)
+    n_view = int(frames.shape[0])
+    rows, cols = (int(x) for x in det_shape)
+    n = N.load().ctlab_brick_plan_ints_3d(*[int(x) for x in shape], n_view)
+    _require(n > 0, f"{name}: bad grid")
+    plan = torch.empty((int(n),), device=frames.device, dtype=torch.int32)
+    with torch.cuda.device(frames.device):
+        N.call("ctlab_brick_plan_3d", N.ptr(frames), n_view, rows, cols, *[int(x) for x in shape], N.ptr(plan),
+               N.stream_of(frames))
+    return plan
+
+
+def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH,
                           out=None, accumulate: bool = False):
     """[B, n_x, n_y, n_z] back projection by LDS-resident bricks (same segments as
-    back_project_3d_on_the_fly_cuda, no global atomics)."""
+    back_project_3d_on_the_fly_cuda, no global atomics).  ``plan`` comes from
+    brick_plan(cone_frames(...))."""
     name = "back_project_3d_brick"
-    _check_f32(name, sino, src, dst, frames, M_inv, b)
+    _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     rows, cols = (int(x) for x in det_shape)
     _require(n_ray % (rows * cols) == 0, f"{name}: n_ray is not a multiple of rows*cols")
     n_view = n_ray // (rows * cols)
-    _require(frames.shape == (n_view, FRAME_FLOATS), f"{name}: frames must be [n_view, {FRAME_FLOATS}]")
+    need = N.load().ctlab_brick_plan_ints_3d(int(n_x), int(n_y), int(n_z), n_view)
+    _require(isinstance(plan, torch.Tensor) and plan.dtype == torch.int32 and plan.numel() == need and plan.is_cuda,
+             f"{name}: plan does not match this geometry")
     if out is None:
         out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
         accumulate = False
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_brick", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
-               N.ptr(frames), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
+               N.ptr(plan), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
                N.ptr(out), N.stream_of(sino))
     return out
 

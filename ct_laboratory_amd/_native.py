@@ -52,12 +52,14 @@ _SIGS = {
     "ctlab_back_project_3d_u16": [_P, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P,
                                   _P],
     "ctlab_cone_frames_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _I64, _I64, _I64, _P, _P],
+    "ctlab_brick_plan_ints_3d": [_I64, _I64, _I64, _I64],
+    "ctlab_brick_plan_3d": [_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_brick": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
                                     _P, _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }
-_RESTYPES = {"ctlab_last_error": ctypes.c_char_p}
+_RESTYPES = {"ctlab_last_error": ctypes.c_char_p, "ctlab_brick_plan_ints_3d": ctypes.c_int64}
 
 
 def exported_symbols() -> list[str]:

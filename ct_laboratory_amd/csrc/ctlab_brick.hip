@@ -2,8 +2,8 @@
 // stack of flat-panel views ([view][rows][cols], one source per view), the
 // layout of the cone-beam and static-ring builders.
 //
-// A workgroup owns a 32^3 brick of the output volume, accumulated in LDS
-// (128 KB), and loops over every view.  Per view it projects the brick's
+// A workgroup owns a 32x32x16 brick of the output volume, accumulated in LDS
+// (70 KB, two workgroups per CU), and loops over every view.  Per view it projects the brick's
 // corners from the view's source onto the detector plane and takes the padded
 // bounding rectangle of detector pixels (the brick's shadow); every ray whose
 // rounded voxels touch the brick lies in that rectangle.  Each such ray is
@@ -21,14 +21,15 @@
 
 namespace ctlab {
 
-constexpr int kBrick = 32;                  // brick edge, voxels
-constexpr int kBrickThreads = 1024;         // 16 waves
-constexpr int kViewChunk = kBrickThreads;   // views whose shadows sit in LDS at once
+constexpr int kBrickX = 32, kBrickY = 32, kBrickZ = 16;  // brick edges, voxels
+constexpr int kBrickThreads = 1024;                       // 16 waves
+constexpr int kViewChunk = 512;  // views whose shadows sit in LDS at once
 constexpr int kFrameFloats = 16;
 // LDS strides of the brick: padded so that 32 lanes stepping along any one
-// axis hit 32 different banks (33 and 32*33+1 are odd multiples mod 32).
-constexpr int kStrideZ = 1, kStrideY = kBrick + 1, kStrideX = kBrick * (kBrick + 1) + 1;
-constexpr int kBrickLds = kBrick * kStrideX;
+// axis hit 32 different banks (17 and 32*17+1 are odd).  About 70 KB, so two
+// workgroups (32 waves, 8 per SIMD) share a CU's 160 KB of LDS.
+constexpr int kStrideZ = 1, kStrideY = kBrickZ + 1, kStrideX = kBrickY * (kBrickZ + 1) + 1;
+constexpr int kBrickLds = kBrickX * kStrideX;
 
 struct Dims3 {
     int n[3];
@@ -179,107 +180,132 @@ __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const in
     return true;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kBrickThreads) void k_back_brick(
-    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
-    int nbx, int nby, int nbz, int n_view, int rows, int cols, int view_chunk, const float* __restrict__ frames,
-    const float* __restrict__ src, const float* __restrict__ dst, const float* __restrict__ minv,
-    const float* __restrict__ b, int accumulate) {
-    __shared__ float acc[kBrickLds];
-    __shared__ int s_r0[kViewChunk], s_w[kViewChunk], s_c0[kViewChunk];
-    __shared__ uint32_t s_pre[kViewChunk + 1];
-    __shared__ uint32_t s_wsum[kBrickThreads / 64];
+// Brick plan: for every brick and view, the shadow rectangle (r0, c0, width)
+// and the exclusive prefix of the rectangle sizes over the views, so that the
+// back projector maps a flat item number to (view, pixel) by binary search.
+// Depends only on the geometry: computed once, reused by every projection.
+constexpr int kPlanInts = 4;  // r0, c0, w, prefix
 
-    const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
-    const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
+struct BrickGrid {
+    int nbx, nby, nbz;
+};
+
+__device__ __forceinline__ Window<3> brick_window(const Dims3& dims, const BrickGrid& bg, int bid) {
+    const int bz = bid % bg.nbz, by = (bid / bg.nbz) % bg.nby, bx = bid / (bg.nbz * bg.nby);
     Window<3> win;
-    win.lo[0] = bx * kBrick;
-    win.lo[1] = by * kBrick;
-    win.lo[2] = bz * kBrick;
-    for (int a = 0; a < 3; ++a) win.ext[a] = min(kBrick, dims.n[a] - win.lo[a]);
+    win.lo[0] = bx * kBrickX;
+    win.lo[1] = by * kBrickY;
+    win.lo[2] = bz * kBrickZ;
+    win.ext[0] = min(kBrickX, dims.n[0] - win.lo[0]);
+    win.ext[1] = min(kBrickY, dims.n[1] - win.lo[1]);
+    win.ext[2] = min(kBrickZ, dims.n[2] - win.lo[2]);
     win.stride[0] = kStrideX;
     win.stride[1] = kStrideY;
     win.stride[2] = kStrideZ;
-    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0.0f;
+    return win;
+}
 
-    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
-    const int64_t bi = blockIdx.y;
-    const float* __restrict__ sn = sino + bi * n_ray;
-
-    for (int v0 = 0; v0 < n_view; v0 += view_chunk) {
-        const int nv = min(view_chunk, n_view - v0);
-        // 1. shadow rectangle of this brick in each view of the chunk
-        uint32_t cnt = 0;
-        if (tid < nv) {
-            int r0, r1, c0, c1;
-            if (!shadow_rect(frames + (int64_t)(v0 + tid) * kFrameFloats, win.lo, win.ext, rows, cols, &r0, &r1,
-                             &c0, &c1)) {
+__global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, int n_view, int rows, int cols,
+                                                    const float* __restrict__ frames, int32_t* __restrict__ plan,
+                                                    uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_wsum[4];
+    const int bid = blockIdx.x;
+    const Window<3> win = brick_window(dims, bg, bid);
+    int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    uint32_t running = 0;
+    for (int v0 = 0; v0 < n_view; v0 += 256) {
+        const int v = v0 + tid;
+        int r0 = 0, r1 = -1, c0 = 0, c1 = -1;
+        if (v < n_view) {
+            if (!shadow_rect(frames + (int64_t)v * kFrameFloats, win.lo, win.ext, rows, cols, &r0, &r1, &c0, &c1)) {
                 r0 = 0;
                 r1 = rows - 1;
                 c0 = 0;
                 c1 = cols - 1;
             }
-            const int h = r1 - r0 + 1, w = c1 - c0 + 1;
-            s_r0[tid] = r0;
-            s_w[tid] = w > 0 ? w : 0;
-            s_c0[tid] = c0;
-            cnt = (h > 0 && w > 0) ? (uint32_t)h * (uint32_t)w : 0u;
         }
-        // 2. exclusive prefix sum of the counts over the block
-        uint32_t x = cnt;
-        const int lane = tid % 64, wv = tid / 64;
+        const int h = r1 - r0 + 1, w = c1 - c0 + 1;
+        const uint32_t cnt = (v < n_view && h > 0 && w > 0) ? (uint32_t)h * (uint32_t)w : 0u;
+        uint32_t x = cnt;  // inclusive scan over the block
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(x, o);
             if (lane >= o) x += y;
         }
         if (lane == 63) s_wsum[wv] = x;
         __syncthreads();
-        if (tid < 64) {
-            uint32_t ws = (lane < kBrickThreads / 64) ? s_wsum[lane] : 0u;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(ws, o);
-                if (lane >= o) ws += y;
-            }
-            if (lane < kBrickThreads / 64) s_wsum[lane] = ws;  // inclusive
+        uint32_t base = running;
+        for (int k = 0; k < wv; ++k) base += s_wsum[k];
+        if (v < n_view) {
+            p[(int64_t)v * kPlanInts + 0] = r0;
+            p[(int64_t)v * kPlanInts + 1] = c0;
+            p[(int64_t)v * kPlanInts + 2] = cnt ? w : 0;
+            p[(int64_t)v * kPlanInts + 3] = (int32_t)(base + x - cnt);
         }
-        __syncthreads();
-        const uint32_t incl = x + (wv > 0 ? s_wsum[wv - 1] : 0u);
-        if (tid < nv) s_pre[tid + 1] = incl;
-        if (tid == 0) s_pre[0] = 0;
-        __syncthreads();
-        const uint32_t total = s_pre[nv];
-
-        // 3. trace every (view, pixel) of the chunk's shadows, columns fastest
-        //    (neighbouring lanes read neighbouring src/dst/sino entries)
-        for (uint64_t item = tid; item < total; item += kBrickThreads) {
-            int lo = 0, hi = nv;  // largest view with s_pre[view] <= item
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= item) lo = mid; else hi = mid;
-            }
-            const uint32_t local = (uint32_t)(item - s_pre[lo]);
-            const uint32_t w = (uint32_t)s_w[lo];
-            const int r = s_r0[lo] + (int)(local / w);
-            const int c = s_c0[lo] + (int)(local % w);
-            const int64_t ray = ((int64_t)(v0 + lo) * rows + r) * cols + c;
-            const float val = sn[ray];
-            if (val == 0.0f) continue;
-            float s[3], d[3];
-            for (int a = 0; a < 3; ++a) {
-                s[a] = src[3 * ray + a];
-                d[a] = dst[3 * ray + a];
-            }
-            auto add = [&](uint32_t li, float len) { atomicAdd(&acc[li], val * len); };
-            if (g.diag)
-                trace_ray_window<3, MODE, true>(g, win, s, d, add);
-            else
-                trace_ray_window<3, MODE, false>(g, win, s, d, add);
-        }
+        running += s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
         __syncthreads();
     }
+    if (tid == 0) totals[bid] = running;
+}
 
-    // 4. write the brick (k fastest: coalesced rows of the volume)
+// The host guarantees total + kBrickThreads < 2^32, so the 32-bit item
+// counter cannot wrap.
+template <int MODE, bool DIAG>
+__device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
+                                            const float* __restrict__ src, const float* __restrict__ dst,
+                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
+                                            uint32_t total, float* acc) {
+    for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
+        int lo = 0, hi = n_view;  // largest view whose prefix <= item
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)p[mid * kPlanInts + 3] <= item) lo = mid; else hi = mid;
+        }
+        const int32_t* e = p + lo * kPlanInts;
+        const uint32_t local = item - (uint32_t)e[3];
+        const uint32_t w = (uint32_t)e[2];
+        const int r = e[0] + (int)(local / w);
+        const int c = e[1] + (int)(local % w);
+        const int64_t ray = ((int64_t)lo * rows + r) * cols + c;
+        const float val = sn[ray];
+        if (val == 0.0f) continue;
+        float s[3], d[3];
+        for (int a = 0; a < 3; ++a) {
+            s[a] = src[3 * ray + a];
+            d[a] = dst[3 * ray + a];
+        }
+        trace_ray_window<3, MODE, DIAG>(g, win, s, d, [&](uint32_t li, float len) { atomicAdd(&acc[li], val * len); });
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
+    const uint32_t* __restrict__ totals, const float* __restrict__ src, const float* __restrict__ dst,
+    const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
+    __shared__ float acc[kBrickLds];
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    const Window<3> win = brick_window(dims, bg, bid);
+    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0.0f;
+    __syncthreads();
+
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const int64_t bi = blockIdx.y;
+    const float* __restrict__ sn = sino + bi * n_ray;
+    const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
+    const uint32_t total = totals[bid];
+
+    // Every (view, pixel) of the brick's shadows, columns fastest: neighbouring
+    // lanes read neighbouring src/dst/sino entries.
+    if (g.diag)
+        brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, acc);
+    else
+        brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, acc);
+    __syncthreads();
+
+    // Write the brick (k fastest: coalesced rows of the volume).
     float* __restrict__ o = out + bi * vol_elems;
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
@@ -311,8 +337,42 @@ extern "C" int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t 
     return check_launch(fn);
 }
 
+static BrickGrid brick_grid(const int64_t* n) {
+    BrickGrid bg;
+    bg.nbx = (int)((n[0] + kBrickX - 1) / kBrickX);
+    bg.nby = (int)((n[1] + kBrickY - 1) / kBrickY);
+    bg.nbz = (int)((n[2] + kBrickZ - 1) / kBrickZ);
+    return bg;
+}
+
+extern "C" int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    if (n_x < 1 || n_y < 1 || n_z < 1 || n_view < 0) return -1;
+    const BrickGrid bg = brick_grid(n);
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * (n_view * kPlanInts + 1);
+}
+
+extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols,
+                                   int64_t n_x, int64_t n_y, int64_t n_z, int32_t* plan, ctlab_stream_t stream) {
+    const char* fn = "brick_plan_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    for (int a = 0; a < 3; ++a) CTLAB_REQUIRE(n[a] >= 1 && n[a] < (1LL << 24), "%s: bad grid dimension", fn);
+    CTLAB_REQUIRE(n_view >= 1 && det_rows >= 1 && det_cols >= 1, "%s: bad detector layout", fn);
+    CTLAB_REQUIRE(n_view * det_rows * det_cols <= 0xffffffffLL - 4096,
+                  "%s: n_view*rows*cols must be below 2^32 - 4096 for the brick plan", fn);
+    CTLAB_REQUIRE(frames && plan, "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nb = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    int32_t* table = plan;
+    uint32_t* totals = reinterpret_cast<uint32_t*>(plan + nb * n_view * kPlanInts);
+    hipLaunchKernelGGL(k_brick_plan, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), d, bg, (int)n_view,
+                       (int)det_rows, (int)det_cols, frames, table, totals);
+    return check_launch(fn);
+}
+
 extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
-                                           int64_t n_view, int64_t det_rows, int64_t det_cols, const float* frames,
+                                           int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
                                            const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
                                            int64_t n_z, int mode, int accumulate, float* out,
                                            ctlab_stream_t stream) {
@@ -328,26 +388,23 @@ extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, con
     if (st) return st;
     CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range", fn, (long long)batch);
     CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
-    CTLAB_REQUIRE(n_view >= 0 && det_rows >= 1 && det_cols >= 1 && det_rows * det_cols <= 0x7fffffffLL,
-                  "%s: bad detector layout", fn);
-    CTLAB_REQUIRE(out, "%s: out is null", fn);
+    CTLAB_REQUIRE(n_view >= 1 && det_rows >= 1 && det_cols >= 1 &&
+                      n_view * det_rows * det_cols <= 0xffffffffLL - 4096,
+                  "%s: bad detector layout (n_view*rows*cols must be below 2^32 - 4096)", fn);
+    CTLAB_REQUIRE(out && sino && src && dst && plan && M_inv && b, "%s: null pointer", fn);
     const int64_t n_ray = n_view * det_rows * det_cols;
-    if (n_ray > 0) CTLAB_REQUIRE(sino && src && dst && frames && M_inv && b, "%s: null pointer", fn);
-    // views per chunk: the chunk's pixel count must fit in 32 bits
-    int64_t chunk = kViewChunk;
-    while (chunk > 1 && chunk * det_rows * det_cols > 0xffffffffLL) chunk /= 2;
-    const int64_t nb = ((n_x + kBrick - 1) / kBrick) * ((n_y + kBrick - 1) / kBrick) * ((n_z + kBrick - 1) / kBrick);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nb = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    const uint32_t* totals = reinterpret_cast<const uint32_t*>(plan + nb * n_view * kPlanInts);
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     dim3 grid((unsigned)nb, (unsigned)batch);
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL(k_back_brick<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, (int)((n_x + kBrick - 1) / kBrick), (int)((n_y + kBrick - 1) / kBrick),
-                           (int)((n_z + kBrick - 1) / kBrick), (int)n_view, (int)det_rows, (int)det_cols, (int)chunk,
-                           frames, src, dst, M_inv, b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, src, dst, M_inv,
+                           b, accumulate);
     else
         hipLaunchKernelGGL(k_back_brick<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, (int)((n_x + kBrick - 1) / kBrick), (int)((n_y + kBrick - 1) / kBrick),
-                           (int)((n_z + kBrick - 1) / kBrick), (int)n_view, (int)det_rows, (int)det_cols, (int)chunk,
-                           frames, src, dst, M_inv, b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, src, dst, M_inv,
+                           b, accumulate);
     return check_launch(fn);
 }

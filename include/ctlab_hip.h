@@ -194,13 +194,22 @@ int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t n_view, int
                          const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, float* frames,
                          ctlab_stream_t stream);
 
+/* Brick plan of a geometry: for every 32x32x16 brick of the grid and every
+ * view, the rectangle of detector pixels that can reach the brick (the
+ * brick's shadow) and the running total of rectangle sizes.
+ * ctlab_brick_plan_ints_3d returns the int32 size of the plan buffer. */
+int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view);
+int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols, int64_t n_x,
+                        int64_t n_y, int64_t n_z, int32_t* plan, ctlab_stream_t stream);
+
 /* Same result as ctlab_back_project_3d_otf (every segment, the same voxel and
- * length), computed by 32^3 bricks accumulated in LDS over all views and
- * written once: no global atomics.  Invalid views are traced over their whole
- * detector (slower, still exact).  out [batch, n_x, n_y, n_z] is overwritten
- * (or added to when accumulate != 0). */
+ * length), computed brick by brick: each workgroup accumulates one brick in
+ * LDS over all views and writes it once, so no global atomics are used.
+ * Views whose frame is invalid are traced over their whole detector (slower,
+ * still exact).  out [batch, n_x, n_y, n_z] is overwritten (or added to when
+ * accumulate != 0). */
 int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
-                                int64_t n_view, int64_t det_rows, int64_t det_cols, const float* frames,
+                                int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
                                 int accumulate, float* out, ctlab_stream_t stream);
 

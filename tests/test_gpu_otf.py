@@ -220,16 +220,18 @@ def test_brick_back_projection(dev, mname, mode):
     ts, td, tm, tb = src.to(dev), dst.to(dev), _t(minv, dev), _t(b, dev)
     fr = _C.cone_frames(ts, td, (52, 40), tm, tb, shape)
     assert bool((fr[:, 15] == 1).all()), "cone-beam views should fit the flat-panel model"
-    got = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), fr, tm, tb, *shape, mode=mode).cpu().numpy()
+    plan = _C.brick_plan(fr, (52, 40), shape)
+    got = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), plan, tm, tb, *shape, mode=mode).cpu().numpy()
     ref = O.back(y, shape, minv, b, src.numpy(), dst.numpy(), mode=mname)
     assert rel_err(got, ref) < 1e-5
     # invalid frames (whole detector per view) give the same result
     fr_bad = fr.clone()
     fr_bad[:, 15] = 0
-    got2 = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), fr_bad, tm, tb, *shape, mode=mode).cpu().numpy()
+    plan_bad = _C.brick_plan(fr_bad, (52, 40), shape)
+    got2 = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), plan_bad, tm, tb, *shape, mode=mode).cpu().numpy()
     assert rel_err(got2, ref) < 1e-5
     # accumulate mode adds into the given volume
     base = torch.ones((2,) + shape, device=dev)
-    acc = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), fr, tm, tb, *shape, mode=mode, out=base,
+    acc = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), plan, tm, tb, *shape, mode=mode, out=base,
                                    accumulate=True).cpu().numpy()
     assert rel_err(acc - 1.0, ref) < 1e-4
