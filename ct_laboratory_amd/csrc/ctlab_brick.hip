@@ -2,15 +2,26 @@
 // stack of flat-panel views ([view][rows][cols], one source per view), the
 // layout of the cone-beam and static-ring builders.
 //
-// A workgroup owns a 32x32x16 brick of the output volume, accumulated in LDS
+// A workgroup owns a 32x16x16 brick of the output volume, accumulated in LDS
 // (70 KB, two workgroups per CU), and loops over every view.  Per view it projects the brick's
 // corners from the view's source onto the detector plane and takes the padded
 // bounding rectangle of detector pixels (the brick's shadow); every ray whose
 // rounded voxels touch the brick lies in that rectangle.  Each such ray is
 // traced with the exact traversal clipped to the brick window
 // (trace_ray_window): the brick receives exactly the segments whose voxel it
-// contains, added with LDS float atomics (ds_add_f32).  The brick is written
-// once with plain stores: no global atomics at all.
+// contains.  The brick is written once with plain stores: no global atomics.
+//
+// Accumulation is 64-bit fixed point (ds_add_u64), because LDS float atomic
+// adds run at a small fraction of integer atomic throughput on gfx950.  Each
+// brick first takes the largest |sinogram| value over its rays; with the
+// longest possible segment (the voxel's body diagonal) and the number of rays
+// this bounds every voxel's sum, and a power-of-two scale maps that bound just
+// below 2^62.  Each float contribution sino*len (the reference's product) is
+// scaled exactly and rounded to an integer: the quantum is 2^-62 of the bound,
+// far below float accumulation error, and integer addition makes the result
+// independent of the order of the rays (deterministic).  Bricks whose rays
+// carry non-finite values fall back to float atomics into global memory so
+// that INF/NaN propagate as in the reference.
 //
 // Exactness does not depend on the detector model: the frame of a view is only
 // used to bound the shadow, conservatively.  A view whose rays do not fit the
@@ -21,13 +32,12 @@
 
 namespace ctlab {
 
-constexpr int kBrickX = 32, kBrickY = 32, kBrickZ = 16;  // brick edges, voxels
+constexpr int kBrickX = 32, kBrickY = 16, kBrickZ = 16;  // brick edges, voxels
 constexpr int kBrickThreads = 1024;                       // 16 waves
-constexpr int kViewChunk = 512;  // views whose shadows sit in LDS at once
 constexpr int kFrameFloats = 16;
-// LDS strides of the brick: padded so that 32 lanes stepping along any one
-// axis hit 32 different banks (17 and 32*17+1 are odd).  About 70 KB, so two
-// workgroups (32 waves, 8 per SIMD) share a CU's 160 KB of LDS.
+// LDS strides of the brick (64-bit cells): padded so that lanes stepping along
+// any one axis hit different banks (17 and 16*17+1 are odd).  About 70 KB, so
+// two workgroups (32 waves, 8 per SIMD) share a CU's 160 KB of LDS.
 constexpr int kStrideZ = 1, kStrideY = kBrickZ + 1, kStrideX = kBrickY * (kBrickZ + 1) + 1;
 constexpr int kBrickLds = kBrickX * kStrideX;
 
@@ -248,25 +258,49 @@ __global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, in
     if (tid == 0) totals[bid] = running;
 }
 
-// The host guarantees total + kBrickThreads < 2^32, so the 32-bit item
-// counter cannot wrap.
-template <int MODE, bool DIAG>
-__device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
-                                            const float* __restrict__ src, const float* __restrict__ dst,
-                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
-                                            uint32_t total, float* acc) {
-    for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
-        int lo = 0, hi = n_view;  // largest view whose prefix <= item
+// Walks the items of a brick in the order one thread visits them (item,
+// item + kBrickThreads, ...): the view of an item never decreases, so after a
+// binary search for the first item the view advances linearly, usually by zero
+// or one step.  The host guarantees total + kBrickThreads < 2^32, so the 32-bit
+// item counter cannot wrap.
+struct BrickItems {
+    const int32_t* __restrict__ p;
+    int n_view, rows, cols;
+    int v;
+
+    __device__ __forceinline__ BrickItems(const int32_t* __restrict__ p_, int n_view_, int rows_, int cols_,
+                                          uint32_t first)
+        : p(p_), n_view(n_view_), rows(rows_), cols(cols_) {
+        int lo = 0, hi = n_view;  // largest view whose prefix <= first
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if ((uint32_t)p[mid * kPlanInts + 3] <= item) lo = mid; else hi = mid;
+            if ((uint32_t)p[mid * kPlanInts + 3] <= first) lo = mid; else hi = mid;
         }
-        const int32_t* e = p + lo * kPlanInts;
+        v = lo;
+    }
+
+    // Ray of an item not below any item passed before.
+    __device__ __forceinline__ int64_t ray(uint32_t item) {
+        while (v + 1 < n_view && (uint32_t)p[(v + 1) * kPlanInts + 3] <= item) ++v;
+        const int32_t* e = p + v * kPlanInts;
         const uint32_t local = item - (uint32_t)e[3];
         const uint32_t w = (uint32_t)e[2];
         const int r = e[0] + (int)(local / w);
         const int c = e[1] + (int)(local % w);
-        const int64_t ray = ((int64_t)lo * rows + r) * cols + c;
+        return ((int64_t)v * rows + r) * cols + c;
+    }
+};
+
+// Traces every item of the brick and hands (window index, contribution) to
+// add().
+template <int MODE, bool DIAG, class Add>
+__device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
+                                            const float* __restrict__ src, const float* __restrict__ dst,
+                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
+                                            uint32_t total, Add&& add) {
+    BrickItems it(p, n_view, rows, cols, threadIdx.x);
+    for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
+        const int64_t ray = it.ray(item);
         const float val = sn[ray];
         if (val == 0.0f) continue;
         float s[3], d[3];
@@ -274,8 +308,39 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
             s[a] = src[3 * ray + a];
             d[a] = dst[3 * ray + a];
         }
-        trace_ray_window<3, MODE, DIAG>(g, win, s, d, [&](uint32_t li, float len) { atomicAdd(&acc[li], val * len); });
+        trace_ray_window<3, MODE, DIAG>(g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); });
     }
+}
+
+// Longest segment any ray can have inside one voxel: the largest body
+// diagonal of the voxel parallelepiped, |M d| for d in {(1,1,1), (1,1,-1),
+// (1,-1,1), (-1,1,1)}, with M the inverse of M_inv.  Evaluated in double.
+__device__ __forceinline__ double voxel_diagonal(const Grid<3>& g) {
+    const float* a = g.minv;
+    const double c00 = (double)a[4] * a[8] - (double)a[5] * a[7], c01 = (double)a[5] * a[6] - (double)a[3] * a[8],
+                 c02 = (double)a[3] * a[7] - (double)a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double M[9];
+    M[0] = c00 / det;
+    M[3] = c01 / det;
+    M[6] = c02 / det;
+    M[1] = ((double)a[2] * a[7] - (double)a[1] * a[8]) / det;
+    M[4] = ((double)a[0] * a[8] - (double)a[2] * a[6]) / det;
+    M[7] = ((double)a[1] * a[6] - (double)a[0] * a[7]) / det;
+    M[2] = ((double)a[1] * a[5] - (double)a[2] * a[4]) / det;
+    M[5] = ((double)a[2] * a[3] - (double)a[0] * a[5]) / det;
+    M[8] = ((double)a[0] * a[4] - (double)a[1] * a[3]) / det;
+    const double ds[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
+    double best = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        double n2 = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            const double v = M[r * 3] * ds[k][0] + M[r * 3 + 1] * ds[k][1] + M[r * 3 + 2] * ds[k][2];
+            n2 += v * v;
+        }
+        best = fmax(best, sqrt(n2));
+    }
+    return best;
 }
 
 template <int MODE>
@@ -284,35 +349,89 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
     const uint32_t* __restrict__ totals, const float* __restrict__ src, const float* __restrict__ dst,
     const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
-    __shared__ float acc[kBrickLds];
+    __shared__ unsigned long long acc[kBrickLds];
+    __shared__ float s_red[kBrickThreads / 64];
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
     const Window<3> win = brick_window(dims, bg, bid);
-    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0.0f;
-    __syncthreads();
+    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
 
     const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
     const int64_t bi = blockIdx.y;
     const float* __restrict__ sn = sino + bi * n_ray;
     const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
     const uint32_t total = totals[bid];
-
-    // Every (view, pixel) of the brick's shadows, columns fastest: neighbouring
-    // lanes read neighbouring src/dst/sino entries.
-    if (g.diag)
-        brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, acc);
-    else
-        brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, acc);
-    __syncthreads();
-
-    // Write the brick (k fastest: coalesced rows of the volume).
     float* __restrict__ o = out + bi * vol_elems;
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
+
+    // 1. largest |value| over the brick's rays (NaN compares false: use a flag)
+    float vmax = 0.0f;
+    bool nonfinite = false;
+    {
+        BrickItems it(p, n_view, rows, cols, tid);
+        for (uint32_t item = tid; item < total; item += kBrickThreads) {
+            const float v = fabsf(sn[it.ray(item)]);
+            nonfinite |= !(v <= 3.0e38f);
+            vmax = fmaxf(vmax, v);
+        }
+    }
+    vmax = nonfinite ? __builtin_huge_valf() : vmax;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o2));
+    if (tid % 64 == 0) s_red[tid / 64] = vmax;
+    __syncthreads();
+    vmax = 0.0f;
+    for (int k = 0; k < kBrickThreads / 64; ++k) vmax = fmaxf(vmax, s_red[k]);
+
+    // 2. power-of-two scale.  A ray's segments in one voxel add up to at most
+    // the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the factor
+    // 2 covers the rounding of len and of the product.  The scale maps that
+    // bound below 2^62 (no overflow of the signed 64-bit sums) and stays a
+    // normal float.  Non-finite data, or products that could overflow a float,
+    // take float atomics into global memory instead (INF/NaN propagate as with
+    // the reference's atomics).
+    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
+    const bool fixed = g.finite && cmax <= 1.0e38;
+    const double bound = cmax * (double)(total > 0 ? total : 1);
+    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;  // bound < 2^e
+    const int F = fixed ? min(62 - e, 126) : 0;
+    const float scale = ldexpf(1.0f, F);
+    if (!fixed) {
+        for (int i = tid; i < nvox; i += kBrickThreads) {
+            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+            const int64_t gi =
+                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+            if (!accumulate) o[gi] = 0.0f;
+        }
+        __syncthreads();
+    }
+
+    // 3. trace and accumulate
+    auto add = [&](uint32_t li, float c) {
+        if (fixed) {
+            atomicAdd(&acc[li], (unsigned long long)__float2ll_rn(c * scale));
+        } else {
+            const int ii = (int)(li / kStrideX), j = (int)((li % kStrideX) / kStrideY), k = (int)(li % kStrideY);
+            const int64_t gi =
+                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+            unsafeAtomicAdd(o + gi, c);
+        }
+    };
+    if (vmax != 0.0f) {
+        if (g.diag)
+            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, add);
+        else
+            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, add);
+    }
+    if (!fixed) return;
+    __syncthreads();
+
+    // 4. write the brick (k fastest: coalesced rows of the volume)
+    const double inv = ldexp(1.0, -F);
     for (int i = tid; i < nvox; i += kBrickThreads) {
         const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
         const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-        const float v = acc[ii * kStrideX + j * kStrideY + k];
+        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
         o[gi] = accumulate ? o[gi] + v : v;
     }
 }

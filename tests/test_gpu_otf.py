@@ -235,3 +235,48 @@ def test_brick_back_projection(dev, mname, mode):
     acc = _C.back_project_3d_brick(_t(y, dev), ts, td, (52, 40), plan, tm, tb, *shape, mode=mode, out=base,
                                    accumulate=True).cpu().numpy()
     assert rel_err(acc - 1.0, ref) < 1e-4
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_brick_signed_nonfinite_deterministic(dev, mname, mode):
+    """Fixed-point brick accumulation: signed data (checked with the A|y| bound),
+    bit-identical repeats, value scales near the float limits, and the float
+    fallback for bricks whose rays carry INF/NaN."""
+    shape = (40, 33, 36)
+    src, dst = build_conebeam_3d_geometry(7, 30, 34, 500.0, 800.0, det_spacing=2.0)
+    b = np.array([-(n - 1) / 2 for n in shape], np.float32)
+    minv = np.eye(3, dtype=np.float32)
+    rng = np.random.default_rng(13)
+    y = rng.standard_normal((1, src.shape[0])).astype(np.float32)
+    ts, td, tm, tb = src.to(dev), dst.to(dev), _t(minv, dev), _t(b, dev)
+    det = (34, 30)
+    plan = _C.brick_plan(_C.cone_frames(ts, td, det, tm, tb, shape), det, shape)
+
+    def brick(v):
+        return _C.back_project_3d_brick(_t(v, dev), ts, td, det, plan, tm, tb, *shape, mode=mode).cpu().numpy()
+
+    got = brick(y)
+    ref = O.back(y, shape, minv, b, src.numpy(), dst.numpy(), mode=mname)
+    bound = O.back(np.abs(y), shape, minv, b, src.numpy(), dst.numpy(), mode=mname)
+    assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
+    assert np.array_equal(got, brick(y)), "fixed-point accumulation must be order independent"
+    for s in (1e-30, 1e30):  # power-of-two-free scales near both ends of the float range
+        ys = (y * np.float32(s)).astype(np.float32)
+        gs = brick(ys)
+        assert np.all(np.abs(gs - ref * s) <= 1e-5 * bound * s + 1e-30 * s)
+    # an INF and a NaN ray: the voxels they cross carry INF / NaN, the rest is unchanged
+    yn = y.copy()
+    k_inf, k_nan = src.shape[0] // 3, 2 * src.shape[0] // 3
+    yn[0, k_inf] = np.inf
+    yn[0, k_nan] = np.nan
+    gn = brick(yn)
+    one = np.zeros_like(y)
+    one[0, k_inf] = 1.0
+    hit_inf = O.back(one, shape, minv, b, src.numpy(), dst.numpy(), mode=mname)[0] > 0
+    one[0, k_inf], one[0, k_nan] = 0.0, 1.0
+    hit_nan = O.back(one, shape, minv, b, src.numpy(), dst.numpy(), mode=mname)[0] > 0
+    assert hit_inf.any() and hit_nan.any()
+    assert np.all(np.isnan(gn[0][hit_nan]))
+    assert np.all(np.isposinf(gn[0][hit_inf & ~hit_nan]))
+    clean = ~(hit_inf | hit_nan)
+    assert np.all(np.abs(gn[0][clean] - ref[0][clean]) <= 1e-5 * bound[0][clean] + 1e-30)
