@@ -292,16 +292,17 @@ struct BrickItems {
 };
 
 // Traces every item of the brick and hands (window index, contribution) to
-// add().
+// add().  Each sinogram value is first multiplied by s1*s2 (powers of two, so
+// exactly, and the product val*len scales with it).
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
                                             const int32_t* __restrict__ p, int n_view, int rows, int cols,
-                                            uint32_t total, Add&& add) {
+                                            uint32_t total, float s1, float s2, Add&& add) {
     BrickItems it(p, n_view, rows, cols, threadIdx.x);
     for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
         const int64_t ray = it.ray(item);
-        const float val = sn[ray];
+        const float val = sn[ray] * s1 * s2;
         if (val == 0.0f) continue;
         float s[3], d[3];
         for (int a = 0; a < 3; ++a) {
@@ -383,19 +384,21 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     vmax = 0.0f;
     for (int k = 0; k < kBrickThreads / 64; ++k) vmax = fmaxf(vmax, s_red[k]);
 
-    // 2. power-of-two scale.  A ray's segments in one voxel add up to at most
-    // the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the factor
-    // 2 covers the rounding of len and of the product.  The scale maps that
-    // bound below 2^62 (no overflow of the signed 64-bit sums) and stays a
-    // normal float.  Non-finite data, or products that could overflow a float,
+    // 2. power-of-two scale 2^F.  A ray's segments in one voxel add up to at
+    // most the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the
+    // factor 2 covers the rounding of len and of the product.  2^F maps that
+    // bound below 2^62 (no overflow of the signed 64-bit sums).  It is applied
+    // to each sinogram value as two float factors, so any F in [-250, 250] is
+    // exact.  The absolute error of a voxel is at most half a quantum 2^-F per
+    // contribution.  Non-finite data, or products that could overflow a float,
     // take float atomics into global memory instead (INF/NaN propagate as with
     // the reference's atomics).
     const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
     const bool fixed = g.finite && cmax <= 1.0e38;
     const double bound = cmax * (double)(total > 0 ? total : 1);
     const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;  // bound < 2^e
-    const int F = fixed ? min(62 - e, 126) : 0;
-    const float scale = ldexpf(1.0f, F);
+    const int F = fixed ? max(-250, min(62 - e, 250)) : 0;
+    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);  // 2^F as two normal floats
     if (!fixed) {
         for (int i = tid; i < nvox; i += kBrickThreads) {
             const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     // 3. trace and accumulate
     auto add = [&](uint32_t li, float c) {
         if (fixed) {
-            atomicAdd(&acc[li], (unsigned long long)__float2ll_rn(c * scale));
+            atomicAdd(&acc[li], (unsigned long long)__float2ll_rn(c));
         } else {
             const int ii = (int)(li / kStrideX), j = (int)((li % kStrideX) / kStrideY), k = (int)(li % kStrideY);
             const int64_t gi =
@@ -419,9 +422,9 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     };
     if (vmax != 0.0f) {
         if (g.diag)
-            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, add);
+            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, s1, s2, add);
         else
-            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, add);
+            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, s1, s2, add);
     }
     if (!fixed) return;
     __syncthreads();
