@@ -414,7 +414,8 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
         if (fixed) {
             atomicAdd(&acc[li], (unsigned long long)__float2ll_rn(c));
         } else {
-            const int ii = (int)(li / kStrideX), j = (int)((li % kStrideX) / kStrideY), k = (int)(li % kStrideY);
+            const uint32_t r = li % kStrideX;
+            const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
             const int64_t gi =
                 ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
             unsafeAtomicAdd(o + gi, c);
