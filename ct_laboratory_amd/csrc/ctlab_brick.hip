@@ -135,12 +135,16 @@ __global__ __launch_bounds__(256) void k_cone_frames(const float* __restrict__ s
 __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const int* ext, int rows, int cols, int* r0,
                             int* r1, int* c0, int* c1) {
     if (f[F_VALID] == 0.0f) return false;
-    // Any ray with a segment whose rounded voxel is in the box passes within E
-    // of it; the model line S -> O + rR + cC of its pixel deviates from the ray
-    // by at most the source and pixel deviations.  Margin: one voxel more.
-    // Float precision is ample: coordinates of ~1e3 voxels err by ~1e-4 voxels,
-    // far inside the margin and the one-pixel pad.
-    const float m = 1.0f + f[F_E] + f[F_DEVS] + f[F_DEVD];
+    // A segment whose rounded voxel is in the box has its computed midpoint in
+    // the box's cells [lo-0.5, lo+ext-0.5], so the ray passes within E (the
+    // midpoint error bound) of them; between source and detector the model
+    // line S -> O + rR + cC of its pixel deviates from the ray by at most the
+    // larger of the source and pixel deviations.  The rays are the pixel
+    // centres, so the pixels needed are the integer points of the projected
+    // box.  Slack for float rounding: 0.05 voxel on the box (the voxel
+    // coordinates of the frame and corners carry errors of ~1e-4) and a
+    // quarter pixel on the rectangle (projection errors of ~1e-4 pixel).
+    const float m = f[F_E] + f[F_DEVS] + f[F_DEVD] + 0.05f;
     const float S[3] = {f[F_S], f[F_S + 1], f[F_S + 2]};
     const float B[3] = {-f[F_R], -f[F_R + 1], -f[F_R + 2]};
     const float Cm[3] = {-f[F_C], -f[F_C + 1], -f[F_C + 2]};
@@ -174,7 +178,7 @@ __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const in
         cmin = fminf(cmin, c);
         cmax = fmaxf(cmax, c);
     }
-    const float pad = 1.0f;
+    const float pad = 0.25f;
     const float fr0 = floorf(rmin - pad), fr1 = ceilf(rmax + pad), fc0 = floorf(cmin - pad), fc1 = ceilf(cmax + pad);
     if (!(fr1 >= 0.0f) || !(fc1 >= 0.0f) || !(fr0 <= (float)(rows - 1)) || !(fc0 <= (float)(cols - 1))) {
         *r0 = 0;
@@ -192,8 +196,9 @@ __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const in
 
 // Brick plan: for every brick and view, the shadow rectangle (r0, c0, width)
 // and the exclusive prefix of the rectangle sizes over the views, so that the
-// back projector maps a flat item number to (view, pixel) by binary search.
-// Depends only on the geometry: computed once, reused by every projection.
+// back projector maps a flat item number to (view, pixel); then the total per
+// brick and the midpoint error bound E of every view (float).  Depends only on
+// the geometry: computed once, reused by every projection.
 constexpr int kPlanInts = 4;  // r0, c0, w, prefix
 
 struct BrickGrid {
@@ -217,9 +222,11 @@ __device__ __forceinline__ Window<3> brick_window(const Dims3& dims, const Brick
 
 __global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, int n_view, int rows, int cols,
                                                     const float* __restrict__ frames, int32_t* __restrict__ plan,
-                                                    uint32_t* __restrict__ totals) {
+                                                    uint32_t* __restrict__ totals, float* __restrict__ view_e) {
     __shared__ uint32_t s_wsum[4];
     const int bid = blockIdx.x;
+    if (bid == 0)
+        for (int v = threadIdx.x; v < n_view; v += blockDim.x) view_e[v] = frames[(int64_t)v * kFrameFloats + F_E];
     const Window<3> win = brick_window(dims, bg, bid);
     int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
     const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
@@ -297,8 +304,9 @@ struct BrickItems {
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
-                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
-                                            uint32_t total, float s1, float s2, Add&& add) {
+                                            const int32_t* __restrict__ p, const float* __restrict__ view_e,
+                                            int n_view, int rows, int cols, uint32_t total, float s1, float s2,
+                                            Add&& add) {
     BrickItems it(p, n_view, rows, cols, threadIdx.x);
     for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
         const int64_t ray = it.ray(item);
@@ -309,7 +317,11 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
             s[a] = src[3 * ray + a];
             d[a] = dst[3 * ray + a];
         }
-        trace_ray_window<3, MODE, DIAG>(g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); });
+        // The view's largest midpoint error bound (with 2^-10 of slack for the
+        // traversal's own rounding of the voxel coordinates), when usable.
+        const float ev = view_e[it.v] * 1.0009765625f;
+        trace_ray_window<3, MODE, DIAG>(
+            g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, (ev <= 8.0f) ? ev : -1.0f);
     }
 }
 
@@ -348,8 +360,8 @@ template <int MODE>
 __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick(
     const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
     BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
-    const uint32_t* __restrict__ totals, const float* __restrict__ src, const float* __restrict__ dst,
-    const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
+    const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ src,
+    const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
     __shared__ unsigned long long acc[kBrickLds];
     __shared__ float s_red[kBrickThreads / 64];
     const int tid = threadIdx.x;
@@ -423,9 +435,9 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     };
     if (vmax != 0.0f) {
         if (g.diag)
-            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, s1, s2, add);
+            brick_items<MODE, true>(g, win, sn, src, dst, p, view_e, n_view, rows, cols, total, s1, s2, add);
         else
-            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, s1, s2, add);
+            brick_items<MODE, false>(g, win, sn, src, dst, p, view_e, n_view, rows, cols, total, s1, s2, add);
     }
     if (!fixed) return;
     __syncthreads();
@@ -472,7 +484,7 @@ extern "C" int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_
     const int64_t n[3] = {n_x, n_y, n_z};
     if (n_x < 1 || n_y < 1 || n_z < 1 || n_view < 0) return -1;
     const BrickGrid bg = brick_grid(n);
-    return (int64_t)bg.nbx * bg.nby * bg.nbz * (n_view * kPlanInts + 1);
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * (n_view * kPlanInts + 1) + n_view;
 }
 
 extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols,
@@ -489,8 +501,9 @@ extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t 
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     int32_t* table = plan;
     uint32_t* totals = reinterpret_cast<uint32_t*>(plan + nb * n_view * kPlanInts);
+    float* view_e = reinterpret_cast<float*>(plan + nb * (n_view * kPlanInts + 1));
     hipLaunchKernelGGL(k_brick_plan, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), d, bg, (int)n_view,
-                       (int)det_rows, (int)det_cols, frames, table, totals);
+                       (int)det_rows, (int)det_cols, frames, table, totals, view_e);
     return check_launch(fn);
 }
 
@@ -519,15 +532,16 @@ extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, con
     const BrickGrid bg = brick_grid(n);
     const int64_t nb = (int64_t)bg.nbx * bg.nby * bg.nbz;
     const uint32_t* totals = reinterpret_cast<const uint32_t*>(plan + nb * n_view * kPlanInts);
+    const float* view_e = reinterpret_cast<const float*>(plan + nb * (n_view * kPlanInts + 1));
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     dim3 grid((unsigned)nb, (unsigned)batch);
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL(k_back_brick<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, src, dst, M_inv,
-                           b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, src, dst,
+                           M_inv, b, accumulate);
     else
         hipLaunchKernelGGL(k_back_brick<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, src, dst, M_inv,
-                           b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, src, dst,
+                           M_inv, b, accumulate);
     return check_launch(fn);
 }

@@ -373,9 +373,14 @@ struct MidIndex {
 // window's bounding planes take the place of the grid's in the clipping
 // argument above, so each segment of the full traversal is visited by exactly
 // the window that contains its voxel.
+//
+// e_bound >= 0 replaces the per-ray midpoint error bound E by a caller's bound
+// that holds for this ray on every axis (e.g. the maximum over a view): a
+// larger E only widens [t_lo, t_hi], which produces the same segments.
 template <int ND, int MODE, bool DIAG, class Visit>
 __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, const Window<ND>& win,
-                                                          const float* src, const float* dst, Visit&& visit) {
+                                                          const float* src, const float* dst, Visit&& visit,
+                                                          float e_bound = -1.0f) {
     if (!g.finite) return;  // the reference's indices are all out of range then
     // Non-finite endpoints make every reference world point INF or NaN, and
     // every such segment is masked (3d_torch.py:233-260): no contribution.
@@ -408,11 +413,14 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
     for (int a = 0; a < ND; ++a) {
         dir[a] = dv[a] - sv[a];
         // Rounding-error bound of a midpoint coordinate on this axis, voxel units.
-        float mag = fabsf(sv[a]) + fabsf(dv[a]) + (float)g.n[a] + 1.0f;
+        float E = e_bound;
+        if (!(E >= 0.0f)) {
+            float mag = fabsf(sv[a]) + fabsf(dv[a]) + (float)g.n[a] + 1.0f;
 #pragma unroll
-        for (int j = 0; j < ND; ++j)
-            mag += fabsf(g.minv[a * ND + j]) * (fabsf(src[j]) + fabsf(dst[j]) + fabsf(g.b[j]));
-        const float E = mag * 7.62939453125e-06f;  // 2^-17
+            for (int j = 0; j < ND; ++j)
+                mag += fabsf(g.minv[a * ND + j]) * (fabsf(src[j]) + fabsf(dst[j]) + fabsf(g.b[j]));
+            E = mag * 7.62939453125e-06f;  // 2^-17
+        }
         // Midpoint coordinates stay within n + 5E of the grid; with E <= 8 they
         // are far below the 2^22 bound of flat_voxel_bounded.  Endpoints that far
         // out (~1e11 voxels) carry no usable float32 crossings anyway.
@@ -437,7 +445,9 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         const int plo = win.lo[a], phi = win.lo[a] + win.ext[a];
         const float t_in = crossing_p<MODE>(up ? plo : phi, sv[a], dir[a], y[a]);
         const float t_out = crossing_p<MODE>(up ? phi : plo, sv[a], dir[a], y[a]);
-        const float tau = 4.0f * E / fabsf(dir[a]) + 9.5367431640625e-07f;  // + 2^-20
+        // 4E/|dir| from y = RN(1/dir): within 2^-23 relative, far inside the
+        // factor-of-two slack of the clipping argument.
+        const float tau = 4.0f * E * fabsf(y[a]) + 9.5367431640625e-07f;  // + 2^-20
         t_lo = fmaxf(t_lo, t_in - tau);
         t_hi = fminf(t_hi, t_out + tau);
     }

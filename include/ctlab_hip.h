@@ -194,9 +194,10 @@ int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t n_view, int
                          const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, float* frames,
                          ctlab_stream_t stream);
 
-/* Brick plan of a geometry: for every 32x32x16 brick of the grid and every
+/* Brick plan of a geometry: for every 32x16x16 brick of the grid and every
  * view, the rectangle of detector pixels that can reach the brick (the
- * brick's shadow) and the running total of rectangle sizes.
+ * brick's shadow) and the running total of rectangle sizes; then the total
+ * per brick and each view's midpoint error bound.
  * ctlab_brick_plan_ints_3d returns the int32 size of the plan buffer. */
 int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view);
 int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols, int64_t n_x,
