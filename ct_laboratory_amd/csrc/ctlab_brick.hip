@@ -298,30 +298,119 @@ struct BrickItems {
     }
 };
 
-// Traces every item of the brick and hands (window index, contribution) to
+// Upper estimate of the number of plane crossings of a ray inside the box
+// [blo, bhi] (voxel coordinates, the brick's cells widened by at least the
+// midpoint error bound), 0 when the ray misses the box on t in [0, 1].  Any
+// ray with a segment whose rounded voxel is in the brick passes through the
+// widened box (its computed midpoint is within E of the exact line), so 0 is
+// returned only for rays the exact traversal would not visit; the extra
+// 2^-14 relative slack covers the rounding of this estimate itself.
+template <int MODE>
+__device__ __forceinline__ uint32_t segment_estimate(const Grid<3>& g, const float* __restrict__ s,
+                                                     const float* __restrict__ d, const float* blo,
+                                                     const float* bhi) {
+    float sv[3], dv[3];
+    to_voxel<3, MODE>(g, s, sv);
+    to_voxel<3, MODE>(g, d, dv);
+    float tin = 0.0f, tout = 1.0f, span = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float dir = dv[a] - sv[a];
+        const float slack = 6.103515625e-05f * (fabsf(sv[a]) + fabsf(dv[a]));
+        const float lo = blo[a] - slack, hi = bhi[a] + slack;
+        if (dir == 0.0f) {
+            if (!(sv[a] >= lo && sv[a] <= hi)) return 0u;
+            continue;
+        }
+        const float r = __builtin_amdgcn_rcpf(dir);
+        const float t1 = (lo - sv[a]) * r, t2 = (hi - sv[a]) * r;
+        tin = fmaxf(tin, fminf(t1, t2));
+        tout = fminf(tout, fmaxf(t1, t2));
+        span += fabsf(dir);
+    }
+    if (!(tin <= tout)) return 0u;
+    return (uint32_t)fminf(span * (tout - tin), 1.0e6f) + 2u;
+}
+
+// Items are traced in chunks of kChunk.  Per chunk every thread takes two
+// items, drops rays with a zero value or no crossing of the brick, and files
+// the rest into buckets by their estimated crossing count (powers of two);
+// then the chunk's rays are traced in bucket order, so the 64 lanes of a wave
+// trace rays of similar length instead of idling behind the longest one.
+constexpr int kChunk = 2 * kBrickThreads;
+constexpr int kBuckets = 8;
+
+// Traces every ray of the brick and hands (window index, contribution) to
 // add().  Each sinogram value is first multiplied by s1*s2 (powers of two, so
-// exactly, and the product val*len scales with it).
+// exactly, and the product val*len scales with it).  e_brick bounds the
+// midpoint error of every ray of the brick (non-finite or above 8: unknown).
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
-                                            const int32_t* __restrict__ p, const float* __restrict__ view_e,
-                                            int n_view, int rows, int cols, uint32_t total, float s1, float s2,
-                                            Add&& add) {
-    BrickItems it(p, n_view, rows, cols, threadIdx.x);
-    for (uint32_t item = threadIdx.x; item < total; item += kBrickThreads) {
-        const int64_t ray = it.ray(item);
-        const float val = sn[ray] * s1 * s2;
-        if (val == 0.0f) continue;
-        float s[3], d[3];
-        for (int a = 0; a < 3; ++a) {
-            s[a] = src[3 * ray + a];
-            d[a] = dst[3 * ray + a];
+                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
+                                            uint32_t total, float e_brick, float s1, float s2,
+                                            uint32_t* queue, uint32_t* bcnt, Add&& add) {
+    const int tid = threadIdx.x;
+    const bool e_ok = e_brick <= 8.0f;
+    const float e_hint = e_ok ? e_brick : -1.0f;
+    const float m = (e_ok ? e_brick : 8.0f) + 0.05f;
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = (float)win.lo[a] - 0.5f - m;
+        bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
+    }
+    BrickItems it(p, n_view, rows, cols, tid);
+    for (uint32_t c0 = 0; c0 < total; c0 += kChunk) {
+        if (tid <= kBuckets) bcnt[tid] = 0u;
+        __syncthreads();
+        uint32_t rk[2], key[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            key[k] = ~0u;
+            rk[k] = 0u;
+            const uint32_t item = c0 + (uint32_t)tid + (uint32_t)(k * kBrickThreads);
+            if (item < total) {
+                const uint32_t ray = (uint32_t)it.ray(item);
+                rk[k] = ray;
+                if (sn[ray] != 0.0f) {
+                    const uint32_t est =
+                        segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
+                    if (est) {
+                        const uint32_t bk = min(32u - (uint32_t)__clz(est - 1u), (uint32_t)(kBuckets - 1));
+                        key[k] = (bk << 12) | atomicAdd(&bcnt[bk], 1u);
+                    }
+                }
+            }
         }
-        // The view's largest midpoint error bound (with 2^-10 of slack for the
-        // traversal's own rounding of the voxel coordinates), when usable.
-        const float ev = view_e[it.v] * 1.0009765625f;
-        trace_ray_window<3, MODE, DIAG>(
-            g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, (ev <= 8.0f) ? ev : -1.0f);
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kBuckets; ++k) {
+                const uint32_t c = bcnt[k];
+                bcnt[k] = run;
+                run += c;
+            }
+            bcnt[kBuckets] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (key[k] != ~0u) queue[bcnt[key[k] >> 12] + (key[k] & 0xfffu)] = rk[k];
+        __syncthreads();
+        const uint32_t hits = bcnt[kBuckets];
+        for (uint32_t q = tid; q < hits; q += kBrickThreads) {
+            const uint32_t ray = queue[q];
+            const float val = sn[ray] * s1 * s2;
+            float s[3], d[3];
+            for (int a = 0; a < 3; ++a) {
+                s[a] = src[3ull * ray + a];
+                d[a] = dst[3ull * ray + a];
+            }
+            trace_ray_window<3, MODE, DIAG>(
+                g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
+        }
+        __syncthreads();
     }
 }
 
@@ -363,6 +452,8 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ src,
     const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
     __shared__ unsigned long long acc[kBrickLds];
+    __shared__ uint32_t queue[kChunk];
+    __shared__ uint32_t bcnt[kBuckets + 1];
     __shared__ float s_red[kBrickThreads / 64];
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
@@ -378,8 +469,10 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
 
-    // 1. largest |value| over the brick's rays (NaN compares false: use a flag)
-    float vmax = 0.0f;
+    // 1. largest |value| over the brick's rays (NaN compares false: use a
+    // flag) and the largest midpoint error bound over the views (INF when one
+    // is not a finite number)
+    float vmax = 0.0f, emax = 0.0f;
     bool nonfinite = false;
     {
         BrickItems it(p, n_view, rows, cols, tid);
@@ -388,13 +481,33 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
             nonfinite |= !(v <= 3.0e38f);
             vmax = fmaxf(vmax, v);
         }
+        for (int v = tid; v < n_view; v += kBrickThreads) {
+            const float ev = view_e[v];
+            emax = fmaxf(emax, (ev >= 0.0f && ev <= 3.0e38f) ? ev : __builtin_huge_valf());
+        }
     }
     vmax = nonfinite ? __builtin_huge_valf() : vmax;
-    for (int o2 = 32; o2 > 0; o2 >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o2));
-    if (tid % 64 == 0) s_red[tid / 64] = vmax;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        vmax = fmaxf(vmax, __shfl_xor(vmax, o2));
+        emax = fmaxf(emax, __shfl_xor(emax, o2));
+    }
+    if (tid % 64 == 0) {
+        s_red[tid / 64] = vmax;
+        queue[tid / 64] = __float_as_uint(emax);
+    }
     __syncthreads();
     vmax = 0.0f;
-    for (int k = 0; k < kBrickThreads / 64; ++k) vmax = fmaxf(vmax, s_red[k]);
+    emax = 0.0f;
+    for (int k = 0; k < kBrickThreads / 64; ++k) {
+        vmax = fmaxf(vmax, s_red[k]);
+        emax = fmaxf(emax, __uint_as_float(queue[k]));
+    }
+    // Both are block-uniform: keep them in scalar registers.
+    vmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmax)));
+    emax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(emax)));
+    // 2^-10 of slack for the traversal's own rounding of the voxel coordinates
+    const float e_brick = emax * 1.0009765625f;
+    __syncthreads();  // queue is reused below
 
     // 2. power-of-two scale 2^F.  A ray's segments in one voxel add up to at
     // most the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the
@@ -435,9 +548,11 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     };
     if (vmax != 0.0f) {
         if (g.diag)
-            brick_items<MODE, true>(g, win, sn, src, dst, p, view_e, n_view, rows, cols, total, s1, s2, add);
+            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, e_brick, s1, s2, queue,
+                                    bcnt, add);
         else
-            brick_items<MODE, false>(g, win, sn, src, dst, p, view_e, n_view, rows, cols, total, s1, s2, add);
+            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, e_brick, s1, s2, queue,
+                                     bcnt, add);
     }
     if (!fixed) return;
     __syncthreads();
