@@ -50,7 +50,7 @@ def build_inputs(dev):
     import torch
 
     from ct_laboratory_amd.phantoms import ellipsoid_phantom_3d
-    from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+    from ct_laboratory_amd.tomography.conebeam_projector_3d import build_conebeam_3d_geometry
 
     c = CFG
     src, dst = build_conebeam_3d_geometry(c["n_view"], c["det_nx"], c["det_ny"], c["sid"], c["sdd"],

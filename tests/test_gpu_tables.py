@@ -7,7 +7,7 @@ import torch
 import oracle as O
 from ct_laboratory_amd import _C
 from ct_laboratory_amd import _native as N
-from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+from ct_laboratory_amd.tomography.conebeam_projector_3d import build_conebeam_3d_geometry
 from helpers import lattice_rays, random_geometry, random_rays, rel_err
 
 pytestmark = pytest.mark.gpu

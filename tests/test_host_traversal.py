@@ -72,7 +72,7 @@ def test_plane_and_corner_rays(host, mode, nd):
 
 @pytest.mark.parametrize("mode", ["torch", "otf"])
 def test_conebeam_rays(host, mode):
-    from ct_laboratory_amd.tomography.conebeam_geometry import build_conebeam_3d_geometry
+    from ct_laboratory_amd.tomography.conebeam_projector_3d import build_conebeam_3d_geometry
 
     src, dst = build_conebeam_3d_geometry(10, 40, 24, 1000.0, 1500.0, det_spacing=1.5 * 256 / 40)
     b = np.full(3, -(256 - 1) / 2, np.float32)
