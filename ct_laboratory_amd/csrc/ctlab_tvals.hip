@@ -171,15 +171,14 @@ __global__ __launch_bounds__(kBlock) void k_compress(const float* __restrict__ t
     if (r >= n_ray) return;
     const float vx = dst[3 * r] - src[3 * r], vy = dst[3 * r + 1] - src[3 * r + 1],
                 vz = dst[3 * r + 2] - src[3 * r + 2];
-    float rl;
+    // Ray length: sqrtf of the squares summed as an fma chain.  nvcc contracts
+    // the reference kernel's sum to it (3d.cu:172-173), and torch.norm(dim=1)
+    // of a float32 [n,3] tensor on the CPU computes the same (3d_torch.py:38-42).
+    float rl = __builtin_sqrtf(__builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)));
     if (SEM == COMPRESS_CUDA) {
-        // sqrtf of the sum of squares as nvcc contracts it (3d.cu:172-173)
-        rl = fmaxf(__builtin_sqrtf(__builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx))), 1e-6f);
+        rl = fmaxf(rl, 1e-6f);
     } else {
-        // torch.norm reduces float32 in double precision (3d_torch.py:38-42)
-        const double ss = (double)vx * vx + (double)vy * vy + (double)vz * vz;
-        rl = (float)sqrt(ss);
-        if (rl < 1e-6f) rl = 1e-6f;
+        if (rl < 1e-6f) rl = 1e-6f;  // clamp(min=1e-6) keeps a NaN
     }
     const float sc = (voxel_diag_len / 10000.0f) / rl;
     scale[r] = sc;

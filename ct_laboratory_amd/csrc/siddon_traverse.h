@@ -96,9 +96,11 @@ __host__ __device__ __forceinline__ Window<ND> full_window(const Grid<ND>& g) {
     return w;
 }
 
-// Reference voxel-space transform of an endpoint: (p - b) @ M_inv^T with
-// separate rounding (3d_torch.py:100-103), or nvcc's contraction of
-// apply_affine_inverse_3d (3d.cu:21-36) in otf mode.
+// Voxel coordinates of a ray end point, (p - b) @ M_inv^T (3d_torch.py:100-103;
+// 2d_torch.py:26-29).  torch's CPU float32 matmul evaluates each 2- or 3-term
+// dot product as the chain fma(x2, m2, fma(x1, m1, x0*m0)), and nvcc contracts
+// the reference's apply_affine_inverse_3d (3d.cu:21-36) to the same chain, so
+// both modes use it.  For a diagonal M_inv every order gives the same value.
 template <int ND, int MODE>
 __host__ __device__ __forceinline__ void to_voxel(const Grid<ND>& g, const float* p, float* out) {
     float sh[ND];
@@ -106,17 +108,10 @@ __host__ __device__ __forceinline__ void to_voxel(const Grid<ND>& g, const float
     for (int j = 0; j < ND; ++j) sh[j] = p[j] - g.b[j];
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        if (MODE == MODE_OTF) {
-            float acc = g.minv[a * ND] * sh[0];
+        float acc = g.minv[a * ND] * sh[0];
 #pragma unroll
-            for (int j = 1; j < ND; ++j) acc = __builtin_fmaf(g.minv[a * ND + j], sh[j], acc);
-            out[a] = acc;
-        } else {
-            float acc = sh[0] * g.minv[a * ND];
-#pragma unroll
-            for (int j = 1; j < ND; ++j) acc = acc + sh[j] * g.minv[a * ND + j];
-            out[a] = acc;
-        }
+        for (int j = 1; j < ND; ++j) acc = __builtin_fmaf(g.minv[a * ND + j], sh[j], acc);
+        out[a] = acc;
     }
 }
 

@@ -80,10 +80,12 @@ def _empty_or_none(t) -> bool:
 
 def voxel_diagonal_length(M: torch.Tensor) -> float:
     """Largest body diagonal |M d| of a voxel, d in {(1,1,1), (1,1,-1), (1,-1,1),
-    (-1,1,1)} (ct_projector_3d_cuda.py:92-104)."""
+    (-1,1,1)} (ct_projector_3d_cuda.py:92-104; ct_projector_3d_torch.py:27-35),
+    evaluated with the same float32 torch operations on the CPU."""
     M, _ = _collapse(M, M.new_zeros(3))
-    d = torch.tensor([[1, 1, 1], [1, 1, -1], [1, -1, 1], [-1, 1, 1]], dtype=M.dtype, device=M.device).t()
-    return float(torch.linalg.vector_norm(M @ d, dim=0).max().item())
+    M = M.detach().to("cpu", torch.float32)
+    d = torch.tensor([[1, 1, 1], [1, 1, -1], [1, -1, 1], [-1, 1, 1]], dtype=torch.float32).t()
+    return float(torch.norm(M @ d, dim=0).max().item())
 
 
 # ---------------------------------------------------------------------------

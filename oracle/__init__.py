@@ -29,17 +29,21 @@ _lib = None
 MODES = {"torch": 0, "otf": 1}
 ZERO_RULES = {"thresh": 0, "exact": 1}
 
-# Which check pins each oracle function (see tests/test_oracle.py).
+# Which check pins each oracle function (tests/test_oracle.py).
 PINNED = {
-    "trace/forward/back torch": "closed forms (reference python_examples), FP64 brute force, "
-    "golden vectors from ct_projector_{2d,3d}_torch.py",
-    "trace/forward/back otf": "closed forms, FP64 brute force on non-degenerate rays "
-    "(nvcc contraction pattern itself: parity unpinned, no nvcc here)",
-    "forward_f64/back_f64": "adjoint identity, chord length A.1, closed forms",
-    "intersections": "golden vectors from compute_intersections_{2d,3d}_torch",
-    "compress torch": "golden vectors from compress_tvals_to_uint16",
-    "compress cuda": "decode-error bound 0.5*scale per crossing; bit pattern parity unpinned",
-    "forward/back tvals, u16": "equal to torch-mode trace on the same crossings",
+    "intersections (2D, 3D)": "golden vectors of compute_intersections_{2d,3d}_torch, bit for bit "
+    "(identity, anisotropic and rotated grids)",
+    "trace/forward/back torch": "golden vectors of forward/back_project_{2d,3d}_torch; closed forms of the "
+    "reference examples; exact rational arithmetic on tiny grids; A.1 = chord length; adjoint identity",
+    "trace_f64/forward_f64/back_f64": "exact rational arithmetic (index lists equal, lengths to 1e-9); "
+    "chord length; adjoint identity to 1e-12",
+    "trace/forward/back otf": "closed forms; A.1 = chord length with the end points inserted; adjoint. "
+    "The exact nvcc contraction of the reference kernel cannot be compiled here: bit patterns of otf "
+    "crossings are parity unpinned",
+    "compress torch, voxel_diag_len": "golden vectors of compress_tvals_to_uint16, bit for bit",
+    "forward/back tvals": "golden projections of the reference's own tables",
+    "compress cuda, forward/back u16": "decode error within half a quantum before the 32000 clamp; "
+    "bit patterns of the CUDA rounding (roundf, FMA decode) parity unpinned",
 }
 
 
@@ -88,6 +92,8 @@ def _declare(L):
     L.orc_compress_cuda.argtypes = [_P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _P]
     L.orc_compress_torch.restype = _INT
     L.orc_compress_torch.argtypes = [_P, _I64, _I64, _P, _P, ctypes.c_float, _P, _P, _P]
+    L.orc_voxel_diag_len.restype = ctypes.c_float
+    L.orc_voxel_diag_len.argtypes = [_P]
     for name in ("orc_forward_u16", "orc_back_u16"):
         f = getattr(L, name)
         f.restype = _INT
@@ -294,11 +300,10 @@ def compress_torch(tvals, src, dst, voxel_diag_len: float):
 
 
 def voxel_diag_len(M) -> np.float32:
-    """max ||M d|| over the four body diagonals, float32 (3d_torch.py:27-35)."""
-    M = np.asarray(M, dtype=np.float32).reshape(3, 3)
-    diags = np.array([[1, 1, 1], [1, 1, -1], [1, -1, 1], [-1, 1, 1]], dtype=np.float32).T
-    v = M @ diags
-    return np.float32(np.sqrt((v * v).sum(axis=0, dtype=np.float32)).max())
+    """max ||M d|| over the four body diagonals, float32, in torch's CPU
+    evaluation order (3d_torch.py:27-35; orc_voxel_diag_len)."""
+    M = np.ascontiguousarray(np.asarray(M, dtype=np.float32).reshape(3, 3))
+    return np.float32(lib().orc_voxel_diag_len(_ptr(M)))
 
 
 def forward_u16(vol, u16, start, scale, minv, b, src, dst):

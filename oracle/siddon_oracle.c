@@ -62,28 +62,28 @@ static int cmp_f64(const void* a, const void* b) {
     return (x < y) ? -1 : (x > y) ? 1 : 0;
 }
 
-/* (p - b) @ M_inv^T, one row, summed left to right with separate rounding
- * (3d_torch.py:100-103).  For a diagonal M_inv this equals (p_a-b_a)*M_inv[a][a]
- * whatever the summation order of a BLAS. */
-static void to_voxel_f32(const orc_geom* g, const float* p, float* out) {
+/* Voxel coordinate of a segment midpoint for the index rule (3d_torch.py:244-254;
+ * 2d_torch.py:120-129): elementwise tensor operations, M_inv[a][j]*shift_j
+ * summed left to right with separate rounding. */
+static void index_coords_f32(const orc_geom* g, const float* m, float* out) {
     int nd = g->ndim;
     float sh[3];
-    for (int j = 0; j < nd; ++j) sh[j] = p[j] - g->b[j];
+    for (int j = 0; j < nd; ++j) sh[j] = m[j] - g->b[j];
     for (int a = 0; a < nd; ++a) {
-        float acc = sh[0] * g->minv[a * nd + 0];
-        for (int j = 1; j < nd; ++j) acc = acc + sh[j] * g->minv[a * nd + j];
+        float acc = g->minv[a * nd + 0] * sh[0];
+        for (int j = 1; j < nd; ++j) acc = acc + g->minv[a * nd + j] * sh[j];
         out[a] = acc;
     }
 }
 
-/* Voxel coordinate of a world point for the index rule (3d_torch.py:244-254). */
-static void index_coords_f32(const orc_geom* g, const float* m, float* out) {
-    to_voxel_f32(g, m, out);
-}
-
-/* CUDA apply_affine_inverse_3d (3d.cu:21-36) as nvcc contracts it:
- * fma(M2, zs, fma(M1, ys, M0*xs)). */
-static void to_voxel_otf(const orc_geom* g, const float* p, float* out) {
+/* Voxel coordinates of a ray end point.  The torch projector forms them as a
+ * matrix product, (p - b) @ M_inv^T (3d_torch.py:100-103; 2d_torch.py:26-29);
+ * torch's CPU float32 matmul evaluates each 2- or 3-term dot product as the
+ * chain fma(x2, m2, fma(x1, m1, x0*m0)) (checked bit for bit against torch, and
+ * pinned by the golden vectors of tests/golden).  The reference's CUDA
+ * apply_affine_inverse_3d (3d.cu:21-36) contracts to the same chain under
+ * nvcc.  For a diagonal M_inv every form gives (p_a - b_a) * M_inv[a][a]. */
+static void to_voxel_mm(const orc_geom* g, const float* p, float* out) {
     int nd = g->ndim;
     float sh[3];
     for (int j = 0; j < nd; ++j) sh[j] = p[j] - g->b[j];
@@ -117,8 +117,8 @@ static int rne_in_range_f32(const orc_geom* g, const float* c, int64_t* ii) {
  * Returns list length (always sum(n_a+1)). */
 static int64_t crossings_torch_f32(const orc_geom* g, const float* src, const float* dst, float* t) {
     float s[3], d[3], dir[3];
-    to_voxel_f32(g, src, s);
-    to_voxel_f32(g, dst, d);
+    to_voxel_mm(g, src, s);
+    to_voxel_mm(g, dst, d);
     int64_t cnt = 0;
     for (int a = 0; a < g->ndim; ++a) {
         dir[a] = d[a] - s[a];
@@ -138,8 +138,8 @@ static int64_t crossings_torch_f32(const orc_geom* g, const float* src, const fl
 /* MODE_OTF list (3d.cu:852-895): kept crossings, plus 0 and 1, sorted. */
 static int64_t crossings_otf_f32(const orc_geom* g, const float* src, const float* dst, float* t) {
     float s[3], d[3];
-    to_voxel_otf(g, src, s);
-    to_voxel_otf(g, dst, d);
+    to_voxel_mm(g, src, s);
+    to_voxel_mm(g, dst, d);
     int64_t cnt = 0;
     for (int a = 0; a < g->ndim; ++a) {
         float v = d[a] - s[a];
@@ -213,7 +213,7 @@ static int64_t trace_f32(const orc_geom* g, int mode, const float* src, const fl
             float tm = 0.5f * (t0 + t1);
             float m[3], c[3];
             for (int a = 0; a < nd; ++a) m[a] = fmaf(tm, vw[a], src[a]);
-            to_voxel_otf(g, m, c);
+            to_voxel_mm(g, m, c);
             int64_t ii[3];
             if (!rne_in_range_f32(g, c, ii)) continue;
             idx_out[nseg] = flat_index(g, ii);
@@ -609,7 +609,7 @@ int orc_compress_cuda(const float* tvals, int64_t n_ray, int64_t K, const float*
 
 /* compress_tvals_to_uint16 (3d_torch.py:4-65): INF -> 10.0, round half even,
  * float running sum, clamp [0, 32000]. voxel_diag_len is passed in (float32,
- * max norm of M applied to the four body diagonals, :27-35). */
+ * max norm of M applied to the four body diagonals, :27-35; orc_voxel_diag_len). */
 int orc_compress_torch(const float* tvals, int64_t n_ray, int64_t K, const float* src, const float* dst,
                        float voxel_diag_len, int16_t* out, float* start, float* scale) {
 #pragma omp parallel for schedule(static)
@@ -617,8 +617,10 @@ int orc_compress_torch(const float* tvals, int64_t n_ray, int64_t K, const float
         const float* s = src + 3 * r;
         const float* d = dst + 3 * r;
         float vx = d[0] - s[0], vy = d[1] - s[1], vz = d[2] - s[2];
-        /* torch.norm reduces float32 with a double accumulator */
-        float rl = (float)sqrt((double)vx * vx + (double)vy * vy + (double)vz * vz);
+        /* torch.norm(dim=1) of a float32 [n,3] tensor on the CPU: the squares
+         * summed as an fma chain, then a correctly rounded sqrtf (checked bit
+         * for bit against torch; pinned by the golden vectors) */
+        float rl = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
         if (rl < 1e-6f) rl = 1e-6f;
         float sc = (voxel_diag_len / 10000.0f) / rl;
         scale[r] = sc;
@@ -746,4 +748,24 @@ void orc_set_num_threads(int n) {
 #else
     (void)n;
 #endif
+}
+
+/* max over the four body diagonals d of ||M d|| (3d_torch.py:27-35) as torch
+ * evaluates it on the CPU in float32: M @ diags with fma-chain dot products,
+ * torch.norm(dim=0) with fma-chain squares and sqrtf, then the maximum. */
+float orc_voxel_diag_len(const float* M) {
+    static const float D[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
+    float best = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float v[3];
+        for (int a = 0; a < 3; ++a) {
+            float acc = M[a * 3 + 0] * D[k][0];
+            acc = fmaf(M[a * 3 + 1], D[k][1], acc);
+            acc = fmaf(M[a * 3 + 2], D[k][2], acc);
+            v[a] = acc;
+        }
+        float n = sqrtf(fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0])));
+        if (n > best) best = n;
+    }
+    return best;
 }
