@@ -92,6 +92,21 @@ def cpu_baseline(src, dst, vol, minv, b, mode_name, n_sample_views=8):
     }
 
 
+def measured_traffic(kernel: str, a) -> float | None:
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the same counters
+    on a copy of known size).  None when no summary matches the configuration."""
+    path = os.path.join(HERE, "profiles", "conebeam256_hbm_traffic.md")
+    if a.mode != "torch" or a.back != "brick" or a.plain_order or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("bench-readable:"):
+                d = json.loads(line.split(":", 1)[1])
+                return d.get(kernel)
+    return None
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +119,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the full-size adjoint check")
     ap.add_argument("--back", choices=["brick", "atomic"], default="brick",
                     help="back projector: LDS bricks (no global atomics) or per-ray atomics")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of 1 GiB (the reference kernel for scaling HBM counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projector shards over rays "
@@ -116,6 +133,14 @@ def main(argv=None) -> int:
     from ct_laboratory_amd import _native as N
 
     dev = torch.device("cuda:0")
+    if a.copy_calibration:
+        x = torch.ones(1 << 28, device=dev)  # 1 GiB of float32
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return 0
     mode = N.MODES[a.mode]
     print(f"[bench] building inputs (cone-beam {CFG})", file=sys.stderr, flush=True)
     src_c, dst_c, vol_c, minv_c, b_c, src, dst, vol, minv, b = build_inputs(dev)
@@ -220,7 +245,7 @@ def main(argv=None) -> int:
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM,
-            "traffic": None,
+            "traffic": measured_traffic(dom, a),
         },
     }
     if not a.no_cpu:

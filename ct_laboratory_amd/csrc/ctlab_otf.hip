@@ -28,12 +28,17 @@ struct RaySched {
     int64_t n_ray;
     uint32_t rows, cols;        // 0: plain order, block b covers rays [256 b, 256 b + 256)
     uint32_t tiles_r, tiles_c;  // ceil(rows / 64), ceil(cols / 4)
+    uint32_t n_view;            // n_ray / (rows * cols)
     uint32_t n_blocks;          // gridDim.x
 };
 
 constexpr int kTileRows = 64;
 constexpr int kTileCols = kBlock / kTileRows;
 constexpr uint32_t kXcds = 8;
+// Consecutive blocks of one XCD take the same detector tile in kViewGroup
+// consecutive views: those rays cross nearly the same voxels, so the volume
+// lines one of them brings into the XCD's L2 serve the others.
+constexpr uint32_t kViewGroup = 8;
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
     const uint32_t q = nb / kXcds, r = nb % kXcds, x = bid % kXcds, j = bid / kXcds;
@@ -46,10 +51,16 @@ __device__ __forceinline__ int64_t ray_of_lane(const RaySched& s) {
         const int64_t r = (int64_t)t * kBlock + threadIdx.x;
         return r < s.n_ray ? r : -1;
     }
+    // t -> (group of kViewGroup views, tile, view within the group); the last
+    // group may hold fewer views.
     const uint32_t per_view = s.tiles_r * s.tiles_c;
-    const uint32_t view = t / per_view, rem = t % per_view;
-    const uint32_t row = (rem / s.tiles_c) * kTileRows + (threadIdx.x % kTileRows);
-    const uint32_t col = (rem % s.tiles_c) * kTileCols + (threadIdx.x / kTileRows);
+    const uint32_t grp = t / (per_view * kViewGroup);
+    const uint32_t rem = t - grp * per_view * kViewGroup;
+    const uint32_t gv = min(kViewGroup, s.n_view - grp * kViewGroup);
+    const uint32_t tile = rem / gv;
+    const uint32_t view = grp * kViewGroup + (rem - tile * gv);
+    const uint32_t row = (tile / s.tiles_c) * kTileRows + (threadIdx.x % kTileRows);
+    const uint32_t col = (tile % s.tiles_c) * kTileCols + (threadIdx.x / kTileRows);
     if (row >= s.rows || col >= s.cols) return -1;
     return ((int64_t)view * s.rows + row) * s.cols + col;
 }
@@ -174,7 +185,7 @@ static Dims<ND> make_dims(const int64_t* n) {
 
 static int make_sched(const char* fn, int64_t n_ray, int64_t rows, int64_t cols, RaySched* s) {
     s->n_ray = n_ray;
-    s->rows = s->cols = s->tiles_r = s->tiles_c = 0;
+    s->rows = s->cols = s->tiles_r = s->tiles_c = s->n_view = 0;
     int64_t blocks;
     if (rows <= 0 && cols <= 0) {
         blocks = (n_ray + kBlock - 1) / kBlock;
@@ -187,6 +198,8 @@ static int make_sched(const char* fn, int64_t n_ray, int64_t rows, int64_t cols,
         s->cols = (uint32_t)cols;
         s->tiles_r = (uint32_t)((rows + kTileRows - 1) / kTileRows);
         s->tiles_c = (uint32_t)((cols + kTileCols - 1) / kTileCols);
+        CTLAB_REQUIRE(n_ray / (rows * cols) <= 0xffffffffLL, "%s: too many views", fn);
+        s->n_view = (uint32_t)(n_ray / (rows * cols));
         blocks = (n_ray / (rows * cols)) * (int64_t)s->tiles_r * s->tiles_c;
     }
     CTLAB_REQUIRE(blocks <= 0xffffffffLL && blocks <= kMaxGridX, "%s: too many rays for one launch", fn);
