@@ -203,12 +203,17 @@ int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t 
 int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols, int64_t n_x,
                         int64_t n_y, int64_t n_z, int32_t* plan, ctlab_stream_t stream);
 
-/* Same result as ctlab_back_project_3d_otf (every segment, the same voxel and
- * length), computed brick by brick: each workgroup accumulates one brick in
- * LDS over all views and writes it once, so no global atomics are used.
+/* Same segments as ctlab_back_project_3d_otf (every segment, the same voxel
+ * and length, each contribution sino*len), computed brick by brick: each
+ * workgroup accumulates one brick in LDS over all views and writes it once, so
+ * no global atomics are used.  Sums are 64-bit fixed point scaled per brick
+ * (quantum 2^-62 of the brick's bound on a voxel sum), so results do not depend
+ * on the order of the rays; bricks whose rays carry INF/NaN use float atomics.
  * Views whose frame is invalid are traced over their whole detector (slower,
  * still exact).  out [batch, n_x, n_y, n_z] is overwritten (or added to when
- * accumulate != 0). */
+ * accumulate != 0).  Alternative schedule of the back projection that
+ * replaces _C.back_project_3d_on_the_fly_cuda (bindings.cpp:101-103) for
+ * rays in the [view][rows][cols] layout. */
 int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
                                 int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
