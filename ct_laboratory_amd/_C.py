@@ -63,12 +63,24 @@ def _sino_dims(name: str, sino: torch.Tensor):
     raise RuntimeError(f"{name}: sino shape!")
 
 
+def _out(name: str, out, shape, device) -> torch.Tensor:
+    """A caller's output buffer (checked) or a new one."""
+    shape = tuple(int(x) for x in shape)
+    if out is None:
+        return torch.empty(shape, device=device, dtype=torch.float32)
+    _check_f32(name, out)
+    _require(tuple(out.shape) == shape, f"{name}: out must have shape {shape}, got {tuple(out.shape)}")
+    _require(out.device == device, f"{name}: out is on another device")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # 3D on-the-fly (bindings.cpp:98-103)
 # ---------------------------------------------------------------------------
 
 
-def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF, det_shape=None):
+def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF, det_shape=None,
+                                       out=None):
     """[B, n_ray] line integrals (src/ct_projector_3d.cu:1018-1049).
 
     ``det_shape=(rows, cols)`` declares rays ordered [view][rows][cols] (a
@@ -78,7 +90,7 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
     n_ray = _rays(name, src, dst, 3)
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
         N.call("ctlab_forward_project_3d_otf", N.ptr(volume), batch, nx, ny, nz, N.ptr(src), N.ptr(dst), n_ray,
@@ -87,14 +99,14 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
 
 
 def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF,
-                                    det_shape=None):
+                                    det_shape=None, out=None):
     """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079)."""
     name = "back_project_3d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
         N.call("ctlab_back_project_3d_otf", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
@@ -108,14 +120,14 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
 
 
 def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N.MODE_OTF,
-                                       zero_rule: int = N.ZERO_THRESH):
+                                       zero_rule: int = N.ZERO_THRESH, out=None):
     """[B, n_ray] line integrals (src/ct_projector_2d.cu:416-440)."""
     name = "forward_project_2d_on_the_fly_cuda"
     _check_f32(name, img, src, dst, M_inv, b)
     batch, (nr, nc) = _vol_dims(name, img, 2)
     n_ray = _rays(name, src, dst, 2)
     _affine(name, M_inv, b, 2)
-    out = torch.empty((batch, n_ray), device=img.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_ray), img.device)
     with torch.cuda.device(img.device):
         N.call("ctlab_forward_project_2d_otf", N.ptr(img), batch, nr, nc, N.ptr(src), N.ptr(dst), n_ray,
                N.ptr(M_inv), N.ptr(b), mode, zero_rule, N.ptr(out), N.stream_of(img))
@@ -123,14 +135,14 @@ def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N
 
 
 def back_project_2d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_row, n_col, *, mode: int = N.MODE_OTF,
-                                    zero_rule: int = N.ZERO_THRESH):
+                                    zero_rule: int = N.ZERO_THRESH, out=None):
     """[B, n_row, n_col] back projection (src/ct_projector_2d.cu:442-465)."""
     name = "back_project_2d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
     _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
     _affine(name, M_inv, b, 2)
-    out = torch.empty((batch, int(n_row), int(n_col)), device=sino.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_row, n_col), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_2d_otf", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
                N.ptr(b), int(n_row), int(n_col), mode, zero_rule, 0, N.ptr(out), N.stream_of(sino))
@@ -228,7 +240,7 @@ def _tvals(name, tvals, n_ray):
     return int(tvals.shape[1])
 
 
-def forward_project_2d_cuda(img, tvals, src, dst, M_inv, b):
+def forward_project_2d_cuda(img, tvals, src, dst, M_inv, b, *, out=None):
     """[B, n_ray] line integrals from a crossing table (src/ct_projector_2d.cu:179-200)."""
     name = "forward_project_2d_cuda"
     _check_f32(name, img, src, dst, M_inv, b)
@@ -236,14 +248,14 @@ def forward_project_2d_cuda(img, tvals, src, dst, M_inv, b):
     n_ray = _rays(name, src, dst, 2)
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 2)
-    out = torch.empty((batch, n_ray), device=img.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_ray), img.device)
     with torch.cuda.device(img.device):
         N.call("ctlab_forward_project_2d_tvals", N.ptr(img), batch, nr, nc, N.ptr(tvals), K, N.ptr(src), N.ptr(dst),
                n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out), N.stream_of(img))
     return out
 
 
-def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col):
+def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col, *, out=None):
     """[B, n_row, n_col] back projection from a crossing table (src/ct_projector_2d.cu:202-221)."""
     name = "back_project_2d_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
@@ -251,14 +263,14 @@ def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col):
     _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 2)
-    out = torch.empty((batch, int(n_row), int(n_col)), device=sino.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_row, n_col), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_2d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
                N.ptr(M_inv), N.ptr(b), int(n_row), int(n_col), 0, N.ptr(out), N.stream_of(sino))
     return out
 
 
-def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b):
+def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b, *, out=None):
     """[B, n_ray] line integrals from a crossing table (src/ct_projector_3d.cu:403-433)."""
     name = "forward_project_3d_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
@@ -266,14 +278,14 @@ def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b):
     n_ray = _rays(name, src, dst, 3)
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         N.call("ctlab_forward_project_3d_tvals", N.ptr(volume), batch, nx, ny, nz, N.ptr(tvals), K, N.ptr(src),
                N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out), N.stream_of(volume))
     return out
 
 
-def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z):
+def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z, *, out=None):
     """[B, n_x, n_y, n_z] back projection from a crossing table (src/ct_projector_3d.cu:435-462)."""
     name = "back_project_3d_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
@@ -281,7 +293,7 @@ def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z):
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
                N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out), N.stream_of(sino))
@@ -321,7 +333,7 @@ def _u16(name, t_uint16, t_start, t_scale, n_ray):
     return int(t_uint16.shape[1])
 
 
-def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, dst, M_inv, b):
+def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, dst, M_inv, b, *, out=None):
     """[B, n_ray] line integrals from a uint16 table (src/ct_projector_3d.cu:491-522)."""
     name = "forward_project_3d_compressed_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
@@ -329,7 +341,7 @@ def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, 
     n_ray = _rays(name, src, dst, 3)
     K = _u16(name, t_uint16, t_start, t_scale, n_ray)
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, n_ray), device=volume.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         N.call("ctlab_forward_project_3d_u16", N.ptr(volume), batch, nx, ny, nz, N.ptr(t_uint16), N.ptr(t_start),
                N.ptr(t_scale), K, N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), N.ptr(out),
@@ -337,7 +349,8 @@ def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, 
     return out
 
 
-def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, M_inv, b, n_x, n_y, n_z):
+def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, M_inv, b, n_x, n_y, n_z, *,
+                                    out=None):
     """[B, n_x, n_y, n_z] back projection from a uint16 table (src/ct_projector_3d.cu:524-553)."""
     name = "back_project_3d_compressed_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
@@ -345,7 +358,7 @@ def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, 
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _u16(name, t_uint16, t_start, t_scale, n_ray)
     _affine(name, M_inv, b, 3)
-    out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_u16", N.ptr(sino), batch, N.ptr(t_uint16), N.ptr(t_start), N.ptr(t_scale), K,
                N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out),
@@ -407,8 +420,8 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
     _require(isinstance(plan, torch.Tensor) and plan.dtype == torch.int32 and plan.numel() == need and plan.is_cuda,
              f"{name}: plan does not match this geometry")
     if out is None:
-        out = torch.empty((batch, int(n_x), int(n_y), int(n_z)), device=sino.device, dtype=torch.float32)
         accumulate = False
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_brick", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
                N.ptr(plan), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),

@@ -31,7 +31,13 @@ COMPRESS_CUDA = 0
 COMPRESS_TORCH = 1
 
 # name -> argtypes (restype is int status unless listed in _RESTYPES)
+_F32 = ctypes.c_float
+
 _SIGS = {
+    "ctlab_map_residual": [_P, _P, _I64, _F32, _P, _P, _P],
+    "ctlab_map_prior_update": [_P, _P, _P, _F32, _F32, _I64, _I64, _I64, _P, _P, _P],
+    "ctlab_map_sqs_weights": [_P, _F32, _F32, _I64, _I64, _I64, _P, _P],
+    "ctlab_zero_f64": [_P, _I64, _P],
     "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I64, _I64, _INT, _P,
                                   _P],

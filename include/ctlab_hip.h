@@ -219,6 +219,33 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
                                 int accumulate, float* out, ctlab_stream_t stream);
 
+/* ---------------------------------------------------------------------- */
+/* MAP iteration with a Gaussian likelihood and the quadratic smoothness   */
+/* prior.  The reference runs it as autograd + torch.optim.SGD            */
+/* (bayesian_estimation/map_estimator.py:113-128, random_variable/        */
+/* gaussian.py:54-63, random_variable/image_energies.py:18-29); these      */
+/* kernels are the parts around the projectors, fused, with no host sync.  */
+/* ---------------------------------------------------------------------- */
+
+/* r = (ax - y) * inv_var; energy[0] += sum 0.5 (ax - y)^2 inv_var (if energy). */
+int ctlab_map_residual(const float* ax, const float* y, int64_t n, float inv_var, float* r, double* energy,
+                       ctlab_stream_t stream);
+
+/* x_out = x - lr * d2 * (bp + beta * L x) on the grid [n0, n1, n2] (n2 fastest;
+ * 2D images use n0 = 1), L = sum over axes of diff^T diff with free ends
+ * (the gradient of (beta/2) sum ||torch.diff(x, dim)||^2); d2 = NULL means 1.
+ * energy[0] += (beta/2) sum ||diff||^2 of x (if energy).  x_out != x. */
+int ctlab_map_prior_update(const float* x, const float* bp, const float* d2, float beta, float lr, int64_t n0,
+                           int64_t n1, int64_t n2, float* x_out, double* energy, ctlab_stream_t stream);
+
+/* Separable quadratic surrogate weights: d2 = 1 / (ata1 * inv_var + 2 beta deg),
+ * ata1 = A^T A 1, deg = number of grid neighbours of the voxel. */
+int ctlab_map_sqs_weights(const float* ata1, float inv_var, float beta, int64_t n0, int64_t n1, int64_t n2,
+                          float* d2, ctlab_stream_t stream);
+
+/* p[0..n) = 0 with a kernel (n <= 256), usable inside a captured graph. */
+int ctlab_zero_f64(double* p, int64_t n, ctlab_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
