@@ -92,6 +92,61 @@ def cpu_baseline(src, dst, vol, minv, b, mode_name, n_sample_views=8):
     }
 
 
+def serve_bench(a, dev) -> int:
+    """Served MAP stream: 70% fan-beam (256^2-512^2, 360-720 views, 2n bins), 30%
+    cone-beam (128^3-256^3, 180-720 views, 256^2-384^2 panel), 20 iterations each
+    with the quadratic prior (weight 1e-2), Poisson arrivals; admission by a 200 GB
+    HBM budget.  Prints one JSON line."""
+    import torch
+
+    from ct_laboratory_amd.serving import ReconServer, build_problem, latency_stats, plain_loop, poisson_workload
+
+    reqs = poisson_workload(a.requests, a.qps)
+    print(f"[serve] building {len(reqs)} problems", file=sys.stderr, flush=True)
+    for r in reqs:
+        build_problem(r, dev)
+    torch.cuda.synchronize()
+    # warm the library and torch kernels once on a small request
+    from ct_laboratory_amd.serving import MAPRequest
+    warm = [MAPRequest(-1, "fan2d", 64, 30, 128, iters=2), MAPRequest(-2, "cone3d", 32, 8, 32, iters=2)]
+    for r in warm:
+        build_problem(r, dev)
+    ReconServer(dev, slots=2).run(warm)
+    plain_loop(warm, dev)
+    print("[serve] server", file=sys.stderr, flush=True)
+    served, span_s = ReconServer(dev, hbm_budget=200e9, slots=a.slots).run(reqs)
+    print("[serve] plain loop", file=sys.stderr, flush=True)
+    plain, span_p = plain_loop(reqs, dev)
+
+    def rmse(res):
+        e = [float(((res[r.rid][0] - r.x_true) ** 2).mean().sqrt()) for r in reqs]
+        return float(np.mean(e))
+
+    def by_kind(res):
+        out = {}
+        for kind in ("fan2d", "cone3d"):
+            sub = {r.rid: res[r.rid] for r in reqs if r.kind == kind}
+            if sub:
+                out[kind] = latency_stats(sub)
+        return out
+
+    worst = max(float((served[r.rid][0] - plain[r.rid][0]).abs().max() / (plain[r.rid][0].abs().max() + 1e-30))
+                for r in reqs)
+    out = {
+        "metric": "served MAP stream: requests/s and p50/p99 latency, continuous batching vs plain per-request loop",
+        "n_requests": len(reqs), "qps": a.qps, "slots": a.slots, "hbm_budget_gb": 200,
+        "mix": {"fan2d": sum(r.kind == "fan2d" for r in reqs), "cone3d": sum(r.kind == "cone3d" for r in reqs)},
+        "server": {"requests_per_s": len(reqs) / span_s, "makespan_s": span_s, **latency_stats(served),
+                   "mean_rmse": rmse(served), "by_kind": by_kind(served)},
+        "plain": {"requests_per_s": len(reqs) / span_p, "makespan_s": span_p, **latency_stats(plain),
+                  "mean_rmse": rmse(plain), "by_kind": by_kind(plain)},
+        "throughput_ratio": span_p / span_s,
+        "max_rel_diff_server_vs_plain": worst,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, a) -> float | None:
     """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the same counters
@@ -121,6 +176,12 @@ def main(argv=None) -> int:
                     help="back projector: LDS bricks (no global atomics) or per-ray atomics")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (the reference kernel for scaling HBM counters)")
+    ap.add_argument("--serve", action="store_true",
+                    help="served MAP stream instead of the projector pair: Poisson arrivals, continuous batching "
+                         "vs the plain per-request loop (p50/p99 latency, throughput)")
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--qps", type=float, default=8.0)
+    ap.add_argument("--slots", type=int, default=8)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projector shards over rays "
@@ -133,6 +194,8 @@ def main(argv=None) -> int:
     from ct_laboratory_amd import _native as N
 
     dev = torch.device("cuda:0")
+    if a.serve:
+        return serve_bench(a, dev)
     if a.copy_calibration:
         x = torch.ones(1 << 28, device=dev)  # 1 GiB of float32
         y = torch.empty_like(x)

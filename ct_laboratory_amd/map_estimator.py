@@ -208,6 +208,9 @@ def sqs_preconditioner(ops: ProjectorOps, var: float, beta: float) -> torch.Tens
 # ---------------------------------------------------------------------------
 
 
+_WARMED = set()
+
+
 class QuadraticMAP:
     """x_{k+1} = x_k - lr * d2 * (A^T (A x_k - y) / var + beta * L x_k), resident on
     the device.  ``d2`` is the squared diagonal preconditioner (None: identity;
@@ -258,23 +261,42 @@ class QuadraticMAP:
         self.cur = 1 - self.cur
         self.iterations += 1
 
+    def _kind(self):
+        ops = self.ops
+        return (ops.nd, type(ops.table).__name__, getattr(ops, "plan", None) is not None, ops.mode, str(ops.device))
+
     def capture(self):
-        """Captures two iterations (buffer 0 -> 1 -> 0) in a HIP graph."""
+        """Captures two iterations (buffer 0 -> 1 -> 0) in a HIP graph.  Capture runs
+        on a side stream with capture_begin/capture_end, which (unlike the
+        torch.cuda.graph context) does not synchronise the device, so other
+        streams keep running.  The first capture of a kind of iteration in the
+        process runs it eagerly once before, so that every kernel has been
+        launched outside a capture."""
         if self.cur != 0:
             self.step()  # the graph starts from buffer 0
         dev = self.ops.device
-        keep = self.x[0].clone()
+        cur = torch.cuda.current_stream(dev)
         side = torch.cuda.Stream(device=dev)
-        side.wait_stream(torch.cuda.current_stream(dev))
-        with torch.cuda.device(dev), torch.cuda.stream(side):
-            self._enqueue(0, self.lr)  # warm-up launches outside the capture
-            self._enqueue(1, self.lr)
-        torch.cuda.current_stream(dev).wait_stream(side)
-        self.x[0].copy_(keep)
+        side.wait_stream(cur)
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.device(dev), torch.cuda.graph(g):
-            self._enqueue(0, self.lr)
-            self._enqueue(1, self.lr)
+        warm = self._kind() not in _WARMED
+        keep = None
+        if warm:
+            keep = self.x[0].clone()
+            keep.record_stream(side)
+        with torch.cuda.device(dev), torch.cuda.stream(side):
+            if warm:
+                self._enqueue(0, self.lr)
+                self._enqueue(1, self.lr)
+                self.x[0].copy_(keep)
+                _WARMED.add(self._kind())
+            g.capture_begin()
+            try:
+                self._enqueue(0, self.lr)
+                self._enqueue(1, self.lr)
+            finally:
+                g.capture_end()
+        cur.wait_stream(side)
         self.graph = g
         return g
 
