@@ -38,6 +38,8 @@ _SIGS = {
     "ctlab_map_prior_update": [_P, _P, _P, _F32, _F32, _I64, _I64, _I64, _P, _P, _P],
     "ctlab_map_sqs_weights": [_P, _F32, _F32, _I64, _I64, _I64, _P, _P],
     "ctlab_zero_f64": [_P, _I64, _P],
+    "ctlab_forward_project_2d_batch": [_P, _I64, _I64, _I64, _I64, _P, _P],
+    "ctlab_back_project_2d_batch": [_P, _I64, _I64, _I64, _I64, _P, _INT, _P],
     "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I64, _I64, _INT, _P,
                                   _P],

@@ -220,6 +220,43 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
                                 int accumulate, float* out, ctlab_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Many 2D scans in one persistent launch (no reference counterpart: the   */
+/* reference projects scans one launch each, ct_projector_2d_cuda.py:23-46) */
+/* ---------------------------------------------------------------------- */
+
+/* One scan.  Pointers are device addresses stored as integers.  For the
+ * forward projection `image` is read and `sino` written; for the back
+ * projection `sino` is read and `image` written.  item_begin = index of the
+ * scan's first work item: scans are listed in item order and scan k owns items
+ * [item_begin_k, item_begin_k + ceil(n_ray_k / rays_per_item)). */
+typedef struct ctlab_scan2d {
+    uint64_t image; /* float [n_row, n_col] */
+    uint64_t sino;  /* float [n_ray] */
+    uint64_t src;   /* float [n_ray, 2] */
+    uint64_t dst;   /* float [n_ray, 2] */
+    float minv[4];  /* row-major inverse of M */
+    float b[2];
+    int32_t n_row, n_col;
+    int64_t n_ray;
+    int32_t mode;      /* CTLAB_MODE_TORCH or CTLAB_MODE_OTF */
+    int32_t zero_rule; /* CTLAB_ZERO_THRESH or CTLAB_ZERO_EXACT */
+    int64_t item_begin;
+} ctlab_scan2d;
+
+/* Forward projections of all scans: `scans` is a device array of n_scan
+ * records, total_items the sum of the scans' item counts, n_groups the number
+ * of persistent workgroups, counter 8 bytes of device memory (cleared by the
+ * call).  Same segments and results as ctlab_forward_project_2d_otf per scan. */
+int ctlab_forward_project_2d_batch(const ctlab_scan2d* scans, int64_t n_scan, int64_t total_items,
+                                   int64_t rays_per_item, int64_t n_groups, void* counter, ctlab_stream_t stream);
+
+/* Back projections of all scans into their images (zeroed first unless
+ * accumulate != 0). */
+int ctlab_back_project_2d_batch(const ctlab_scan2d* scans, int64_t n_scan, int64_t total_items,
+                                int64_t rays_per_item, int64_t n_groups, void* counter, int accumulate,
+                                ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* MAP iteration with a Gaussian likelihood and the quadratic smoothness   */
 /* prior.  The reference runs it as autograd + torch.optim.SGD            */
 /* (bayesian_estimation/map_estimator.py:113-128, random_variable/        */

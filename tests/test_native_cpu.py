@@ -42,3 +42,12 @@ def test_argument_errors_are_reported_without_gpu():
     assert st == 1 and b"mode" in L.ctlab_last_error()
     st = L.ctlab_forward_project_2d_otf(None, 0, 4, 4, None, None, 1, None, None, 0, 0, None, None)
     assert st == 1 and b"batch" in L.ctlab_last_error()
+
+
+def test_scan2d_record_layout():
+    """The numpy record matches the C struct ctlab_scan2d field by field."""
+    from ct_laboratory_amd.batched import SCAN_DTYPE
+
+    expect = {"image": 0, "sino": 8, "src": 16, "dst": 24, "minv": 32, "b": 48, "n_row": 56, "n_col": 60,
+              "n_ray": 64, "mode": 72, "zero_rule": 76, "item_begin": 80}
+    assert {k: SCAN_DTYPE.fields[k][1] for k in expect} == expect and SCAN_DTYPE.itemsize == 88
