@@ -2,8 +2,8 @@
 // stack of flat-panel views ([view][rows][cols], one source per view), the
 // layout of the cone-beam and static-ring builders.
 //
-// A workgroup owns a 32x16x16 brick of the output volume, accumulated in LDS
-// (70 KB, two workgroups per CU), and loops over every view.  Per view it projects the brick's
+// A workgroup owns a 32x32x8 brick of the output volume, accumulated in LDS
+// (64 KB, two workgroups per CU), and loops over every view.  Per view it projects the brick's
 // corners from the view's source onto the detector plane and takes the padded
 // bounding rectangle of detector pixels (the brick's shadow); every ray whose
 // rounded voxels touch the brick lies in that rectangle.  Each such ray is
@@ -32,13 +32,15 @@
 
 namespace ctlab {
 
-constexpr int kBrickX = 32, kBrickY = 16, kBrickZ = 16;  // brick edges, voxels
-constexpr int kBrickThreads = 1024;                       // 16 waves
+constexpr int kBrickX = 32, kBrickY = 32, kBrickZ = 8;  // brick edges, voxels
+constexpr int kBrickThreads = 1024;                      // 16 waves
 constexpr int kFrameFloats = 16;
-// LDS strides of the brick (64-bit cells): padded so that lanes stepping along
-// any one axis hit different banks (17 and 16*17+1 are odd).  About 70 KB, so
-// two workgroups (32 waves, 8 per SIMD) share a CU's 160 KB of LDS.
-constexpr int kStrideZ = 1, kStrideY = kBrickZ + 1, kStrideX = kBrickY * (kBrickZ + 1) + 1;
+// LDS strides of the brick (64-bit cells); the x stride 32*8+1 is odd so that
+// lanes stepping along x hit different banks.  About 64 KB, so two workgroups
+// (32 waves, 8 per SIMD) with their ray queues share a CU's 160 KB of LDS.
+// Rays of a circular scan run mostly in the x-y plane: a brick long in x and y
+// and thin in z is entered by fewer rays per segment than a cubic one.
+constexpr int kStrideZ = 1, kStrideY = kBrickZ, kStrideX = kBrickY * kBrickZ + 1;
 constexpr int kBrickLds = kBrickX * kStrideX;
 
 struct Dims3 {

@@ -194,7 +194,7 @@ int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t n_view, int
                          const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, float* frames,
                          ctlab_stream_t stream);
 
-/* Brick plan of a geometry: for every 32x16x16 brick of the grid and every
+/* Brick plan of a geometry: for every 32x32x8 brick of the grid and every
  * view, the rectangle of detector pixels that can reach the brick (the
  * brick's shadow) and the running total of rectangle sizes; then the total
  * per brick and each view's midpoint error bound.
