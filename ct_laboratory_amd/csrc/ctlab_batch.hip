@@ -45,9 +45,13 @@ __device__ __forceinline__ void item_rays(const ctlab_scan2d& sc, const Grid<2>&
             trace_ray<2, MODE, DIAG>(g, s, d, [&](uint32_t idx, float len) { unsafeAtomicAdd(img + idx, val * len); });
         } else {
             const float* __restrict__ img = reinterpret_cast<const float*>(sc.image);
-            float acc = 0.0f;
-            trace_ray<2, MODE, DIAG>(g, s, d, [&](uint32_t idx, float len) { acc = __builtin_fmaf(img[idx], len, acc); });
-            reinterpret_cast<float*>(sc.sino)[r] = acc;
+            float acc = 0.0f, pv = 0.0f, pl = 0.0f;  // gather consumed one segment later
+            trace_ray<2, MODE, DIAG>(g, s, d, [&](uint32_t idx, float len) {
+                acc = __builtin_fmaf(pv, pl, acc);
+                pv = img[idx];
+                pl = len;
+            });
+            reinterpret_cast<float*>(sc.sino)[r] = __builtin_fmaf(pv, pl, acc);
         }
     }
 }

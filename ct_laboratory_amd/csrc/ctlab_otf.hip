@@ -90,13 +90,20 @@ __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict_
     float s[ND], d[ND];
     load_ray<ND>(src, dst, r, s, d);
     const float* __restrict__ v = vol + bi * vol_elems;
-    float acc = 0.0f;
-    auto gather = [&](uint32_t idx, float len) { acc = __builtin_fmaf(v[idx], len, acc); };
+    // The gather of a segment is consumed at the next visited segment, so its
+    // memory latency overlaps the next crossing and midpoint computation.  The
+    // sum is formed in the same order (acc = fma(value, len, acc) per segment).
+    float acc = 0.0f, pv = 0.0f, pl = 0.0f;
+    auto gather = [&](uint32_t idx, float len) {
+        acc = __builtin_fmaf(pv, pl, acc);
+        pv = v[idx];
+        pl = len;
+    };
     if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, gather);
     else
         trace_ray<ND, MODE, false>(g, s, d, gather);
-    out[bi * n_ray + r] = acc;
+    out[bi * n_ray + r] = __builtin_fmaf(pv, pl, acc);
 }
 
 template <int ND, int MODE>
