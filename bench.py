@@ -286,6 +286,7 @@ def main(argv=None) -> int:
     if a.back == "brick":
         frames = _C.cone_frames(src, dst, (CFG["det_ny"], CFG["det_nx"]), minv, b, (n, n, n))
         plan = _C.brick_plan(frames, (CFG["det_ny"], CFG["det_nx"]), (n, n, n))
+        ws = torch.empty(CFG["n_view"], device=dev)
         n_valid = int((frames[:, 15] == 1).sum().item())
         print(f"[bench] detector frames: {n_valid}/{frames.shape[0]} views fit the flat-panel model",
               file=sys.stderr, flush=True)
@@ -293,6 +294,7 @@ def main(argv=None) -> int:
     def back(y):
         if a.back == "brick":
             return _C.back_project_3d_brick(y, src, dst, (CFG["det_ny"], CFG["det_nx"]), plan, minv, b, n, n, n,
+                                            workspace=ws,
                                             mode=mode)
         return _C.back_project_3d_on_the_fly_cuda(y, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
 

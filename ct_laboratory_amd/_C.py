@@ -405,7 +405,7 @@ def brick_plan(frames, det_shape, shape):
 
 
 def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH,
-                          out=None, accumulate: bool = False):
+                          out=None, accumulate: bool = False, workspace=None):
     """[B, n_x, n_y, n_z] back projection by LDS-resident bricks (same segments as
     back_project_3d_on_the_fly_cuda, no global atomics).  ``plan`` comes from
     brick_plan(cone_frames(...))."""
@@ -422,10 +422,14 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
     if out is None:
         accumulate = False
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
+    if workspace is None:
+        workspace = torch.empty(batch * n_view, device=sino.device, dtype=torch.float32)
+    _require(workspace.dtype == torch.float32 and workspace.is_cuda and workspace.numel() >= batch * n_view,
+             f"{name}: workspace must hold batch * n_view floats")
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_brick", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
                N.ptr(plan), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
-               N.ptr(out), N.stream_of(sino))
+               N.ptr(out), N.ptr(workspace), N.stream_of(sino))
     return out
 
 

@@ -63,7 +63,7 @@ _SIGS = {
     "ctlab_brick_plan_ints_3d": [_I64, _I64, _I64, _I64],
     "ctlab_brick_plan_3d": [_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_brick": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
-                                    _P, _P],
+                                    _P, _P, _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }

@@ -199,8 +199,9 @@ __device__ bool shadow_rect(const float* __restrict__ f, const int* lo, const in
 // Brick plan: for every brick and view, the shadow rectangle (r0, c0, width)
 // and the exclusive prefix of the rectangle sizes over the views, so that the
 // back projector maps a flat item number to (view, pixel); then the total per
-// brick and the midpoint error bound E of every view (float).  Depends only on
-// the geometry: computed once, reused by every projection.
+// brick, the midpoint error bound E of every view (float) and a copy of the
+// views' frames.  Depends only on the geometry: computed once, reused by every
+// projection.
 constexpr int kPlanInts = 4;  // r0, c0, w, prefix
 
 struct BrickGrid {
@@ -224,11 +225,14 @@ __device__ __forceinline__ Window<3> brick_window(const Dims3& dims, const Brick
 
 __global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, int n_view, int rows, int cols,
                                                     const float* __restrict__ frames, int32_t* __restrict__ plan,
-                                                    uint32_t* __restrict__ totals, float* __restrict__ view_e) {
+                                                    uint32_t* __restrict__ totals, float* __restrict__ view_e,
+                                                    float* __restrict__ frames_copy) {
     __shared__ uint32_t s_wsum[4];
     const int bid = blockIdx.x;
-    if (bid == 0)
+    if (bid == 0) {
         for (int v = threadIdx.x; v < n_view; v += blockDim.x) view_e[v] = frames[(int64_t)v * kFrameFloats + F_E];
+        for (int64_t i = threadIdx.x; i < (int64_t)n_view * kFrameFloats; i += blockDim.x) frames_copy[i] = frames[i];
+    }
     const Window<3> win = brick_window(dims, bg, bid);
     int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
     const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, in
 struct BrickItems {
     const int32_t* __restrict__ p;
     int n_view, rows, cols;
-    int v;
+    int v, r, c;  // view, detector row and column of the last item
 
     __device__ __forceinline__ BrickItems(const int32_t* __restrict__ p_, int n_view_, int rows_, int cols_,
                                           uint32_t first)
@@ -294,8 +298,8 @@ struct BrickItems {
         const int32_t* e = p + v * kPlanInts;
         const uint32_t local = item - (uint32_t)e[3];
         const uint32_t w = (uint32_t)e[2];
-        const int r = e[0] + (int)(local / w);
-        const int c = e[1] + (int)(local % w);
+        r = e[0] + (int)(local / w);
+        c = e[1] + (int)(local % w);
         return ((int64_t)v * rows + r) * cols + c;
     }
 };
@@ -307,6 +311,29 @@ struct BrickItems {
 // widened box (its computed midpoint is within E of the exact line), so 0 is
 // returned only for rays the exact traversal would not visit; the extra
 // 2^-14 relative slack covers the rounding of this estimate itself.
+__device__ __forceinline__ uint32_t slab_estimate(const float* sv, const float* dv, const float* blo,
+                                                  const float* bhi, float extra) {
+    float tin = 0.0f, tout = 1.0f, span = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float dir = dv[a] - sv[a];
+        const float slack = extra + 6.103515625e-05f * (fabsf(sv[a]) + fabsf(dv[a]));
+        const float lo = blo[a] - slack, hi = bhi[a] + slack;
+        if (dir == 0.0f) {
+            if (!(sv[a] >= lo && sv[a] <= hi)) return 0u;
+            continue;
+        }
+        const float rc = __builtin_amdgcn_rcpf(dir);
+        const float t1 = (lo - sv[a]) * rc, t2 = (hi - sv[a]) * rc;
+        tin = fmaxf(tin, fminf(t1, t2));
+        tout = fminf(tout, fmaxf(t1, t2));
+        span += fabsf(dir);
+    }
+    if (!(tin <= tout)) return 0u;
+    return (uint32_t)fminf(span * (tout - tin), 1.0e6f) + 2u;
+}
+
+// From the ray's own end points.
 template <int MODE>
 __device__ __forceinline__ uint32_t segment_estimate(const Grid<3>& g, const float* __restrict__ s,
                                                      const float* __restrict__ d, const float* blo,
@@ -314,24 +341,20 @@ __device__ __forceinline__ uint32_t segment_estimate(const Grid<3>& g, const flo
     float sv[3], dv[3];
     to_voxel<3, MODE>(g, s, sv);
     to_voxel<3, MODE>(g, d, dv);
-    float tin = 0.0f, tout = 1.0f, span = 0.0f;
+    return slab_estimate(sv, dv, blo, bhi, 0.0f);
+}
+
+// From the view's flat-panel frame: the model line from the view's source to
+// O + r R + c C deviates from the ray by at most the source and pixel
+// deviations, which widen the box (the same float expression as the frame's
+// fit, so the bound holds for these very values).
+__device__ __forceinline__ uint32_t frame_estimate(const float* __restrict__ f, int r, int c, const float* blo,
+                                                   const float* bhi) {
+    const float sv[3] = {f[F_S], f[F_S + 1], f[F_S + 2]};
+    float dv[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float dir = dv[a] - sv[a];
-        const float slack = 6.103515625e-05f * (fabsf(sv[a]) + fabsf(dv[a]));
-        const float lo = blo[a] - slack, hi = bhi[a] + slack;
-        if (dir == 0.0f) {
-            if (!(sv[a] >= lo && sv[a] <= hi)) return 0u;
-            continue;
-        }
-        const float r = __builtin_amdgcn_rcpf(dir);
-        const float t1 = (lo - sv[a]) * r, t2 = (hi - sv[a]) * r;
-        tin = fmaxf(tin, fminf(t1, t2));
-        tout = fminf(tout, fmaxf(t1, t2));
-        span += fabsf(dir);
-    }
-    if (!(tin <= tout)) return 0u;
-    return (uint32_t)fminf(span * (tout - tin), 1.0e6f) + 2u;
+    for (int a = 0; a < 3; ++a) dv[a] = f[F_O + a] + (float)r * f[F_R + a] + (float)c * f[F_C + a];
+    return slab_estimate(sv, dv, blo, bhi, f[F_DEVS] + f[F_DEVD]);
 }
 
 // Items are traced in chunks of kChunk.  Per chunk every thread takes two
@@ -349,9 +372,9 @@ constexpr int kBuckets = 8;
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
-                                            const int32_t* __restrict__ p, int n_view, int rows, int cols,
-                                            uint32_t total, float e_brick, float s1, float s2,
-                                            uint32_t* queue, uint32_t* bcnt, Add&& add) {
+                                            const int32_t* __restrict__ p, const float* __restrict__ frames,
+                                            int n_view, int rows, int cols, uint32_t total, float e_brick, float s1,
+                                            float s2, uint32_t* queue, uint32_t* bcnt, Add&& add) {
     const int tid = threadIdx.x;
     const bool e_ok = e_brick <= 8.0f;
     const float e_hint = e_ok ? e_brick : -1.0f;
@@ -376,8 +399,10 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
                 const uint32_t ray = (uint32_t)it.ray(item);
                 rk[k] = ray;
                 if (sn[ray] != 0.0f) {
-                    const uint32_t est =
-                        segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
+                    const float* __restrict__ f = frames + (int64_t)it.v * kFrameFloats;
+                    const uint32_t est = (f[F_VALID] != 0.0f)
+                                             ? frame_estimate(f, it.r, it.c, blo, bhi)
+                                             : segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
                     if (est) {
                         const uint32_t bk = min(32u - (uint32_t)__clz(est - 1u), (uint32_t)(kBuckets - 1));
                         key[k] = (bk << 12) | atomicAdd(&bcnt[bk], 1u);
@@ -447,12 +472,35 @@ __device__ __forceinline__ double voxel_diagonal(const Grid<3>& g) {
     return best;
 }
 
+// Largest |value| of each view's sinogram block (one workgroup per view and
+// batch element); INF when the view holds an INF or NaN.
+__global__ __launch_bounds__(256) void k_view_absmax(const float* __restrict__ sino, int64_t per_view, int n_view,
+                                                     float* __restrict__ view_max) {
+    __shared__ float red[4];
+    const int64_t base = ((int64_t)blockIdx.y * n_view + blockIdx.x) * per_view;
+    float m = 0.0f;
+    bool bad = false;
+    for (int64_t i = threadIdx.x; i < per_view; i += blockDim.x) {
+        const float a = fabsf(sino[base + i]);
+        bad |= !(a <= 3.0e38f);
+        m = fmaxf(m, a);
+    }
+    m = bad ? __builtin_huge_valf() : m;
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        view_max[(int64_t)blockIdx.y * n_view + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick(
     const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
     BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
-    const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ src,
-    const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b, int accumulate) {
+    const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ frames,
+    const float* __restrict__ src,
+    const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b,
+    const float* __restrict__ view_max, int accumulate) {
     __shared__ unsigned long long acc[kBrickLds];
     __shared__ uint32_t queue[kChunk];
     __shared__ uint32_t bcnt[kBuckets + 1];
@@ -471,19 +519,20 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
 
-    // 1. largest |value| over the brick's rays (NaN compares false: use a
-    // flag) and the largest midpoint error bound over the views (INF when one
-    // is not a finite number)
+    // 1. a bound on |value| over the brick's rays: the largest per-view
+    // maximum (k_view_absmax, INF for a view holding INF or NaN) over the views
+    // whose shadow rectangle is not empty; and the largest midpoint error bound
+    // over the views (INF when one is not a finite number)
     float vmax = 0.0f, emax = 0.0f;
     bool nonfinite = false;
     {
-        BrickItems it(p, n_view, rows, cols, tid);
-        for (uint32_t item = tid; item < total; item += kBrickThreads) {
-            const float v = fabsf(sn[it.ray(item)]);
-            nonfinite |= !(v <= 3.0e38f);
-            vmax = fmaxf(vmax, v);
-        }
+        const float* __restrict__ vm = view_max + bi * n_view;
         for (int v = tid; v < n_view; v += kBrickThreads) {
+            if (p[(int64_t)v * kPlanInts + 2] > 0) {
+                const float m = vm[v];
+                nonfinite |= !(m <= 3.0e38f);
+                vmax = fmaxf(vmax, m);
+            }
             const float ev = view_e[v];
             emax = fmaxf(emax, (ev >= 0.0f && ev <= 3.0e38f) ? ev : __builtin_huge_valf());
         }
@@ -550,11 +599,11 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     };
     if (vmax != 0.0f) {
         if (g.diag)
-            brick_items<MODE, true>(g, win, sn, src, dst, p, n_view, rows, cols, total, e_brick, s1, s2, queue,
-                                    bcnt, add);
+            brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1, s2,
+                                    queue, bcnt, add);
         else
-            brick_items<MODE, false>(g, win, sn, src, dst, p, n_view, rows, cols, total, e_brick, s1, s2, queue,
-                                     bcnt, add);
+            brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1, s2,
+                                     queue, bcnt, add);
     }
     if (!fixed) return;
     __syncthreads();
@@ -601,7 +650,7 @@ extern "C" int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_
     const int64_t n[3] = {n_x, n_y, n_z};
     if (n_x < 1 || n_y < 1 || n_z < 1 || n_view < 0) return -1;
     const BrickGrid bg = brick_grid(n);
-    return (int64_t)bg.nbx * bg.nby * bg.nbz * (n_view * kPlanInts + 1) + n_view;
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * (n_view * kPlanInts + 1) + n_view * (1 + kFrameFloats);
 }
 
 extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols,
@@ -619,15 +668,16 @@ extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t 
     int32_t* table = plan;
     uint32_t* totals = reinterpret_cast<uint32_t*>(plan + nb * n_view * kPlanInts);
     float* view_e = reinterpret_cast<float*>(plan + nb * (n_view * kPlanInts + 1));
+    float* frames_copy = view_e + n_view;
     hipLaunchKernelGGL(k_brick_plan, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), d, bg, (int)n_view,
-                       (int)det_rows, (int)det_cols, frames, table, totals, view_e);
+                       (int)det_rows, (int)det_cols, frames, table, totals, view_e, frames_copy);
     return check_launch(fn);
 }
 
 extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
                                            int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
                                            const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
-                                           int64_t n_z, int mode, int accumulate, float* out,
+                                           int64_t n_z, int mode, int accumulate, float* out, float* view_max,
                                            ctlab_stream_t stream) {
     const char* fn = "back_project_3d_brick";
     const int64_t n[3] = {n_x, n_y, n_z};
@@ -644,21 +694,26 @@ extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, con
     CTLAB_REQUIRE(n_view >= 1 && det_rows >= 1 && det_cols >= 1 &&
                       n_view * det_rows * det_cols <= 0xffffffffLL - 4096,
                   "%s: bad detector layout (n_view*rows*cols must be below 2^32 - 4096)", fn);
-    CTLAB_REQUIRE(out && sino && src && dst && plan && M_inv && b, "%s: null pointer", fn);
+    CTLAB_REQUIRE(out && sino && src && dst && plan && M_inv && b && view_max, "%s: null pointer", fn);
     const int64_t n_ray = n_view * det_rows * det_cols;
     const BrickGrid bg = brick_grid(n);
     const int64_t nb = (int64_t)bg.nbx * bg.nby * bg.nbz;
     const uint32_t* totals = reinterpret_cast<const uint32_t*>(plan + nb * n_view * kPlanInts);
     const float* view_e = reinterpret_cast<const float*>(plan + nb * (n_view * kPlanInts + 1));
+    const float* frames = view_e + n_view;
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     dim3 grid((unsigned)nb, (unsigned)batch);
+    hipLaunchKernelGGL(k_view_absmax, dim3((unsigned)n_view, (unsigned)batch), dim3(256), 0, as_stream(stream), sino,
+                       det_rows * det_cols, (int)n_view, view_max);
+    st = check_launch(fn);
+    if (st) return st;
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL(k_back_brick<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, src, dst,
-                           M_inv, b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, frames, src,
+                           dst, M_inv, b, view_max, accumulate);
     else
         hipLaunchKernelGGL(k_back_brick<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, src, dst,
-                           M_inv, b, accumulate);
+                           n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, frames, src,
+                           dst, M_inv, b, view_max, accumulate);
     return check_launch(fn);
 }

@@ -140,6 +140,10 @@ class ProjectorOps:
             opts = proj._options() if self.table is None else None
             self.det_shape = None if opts is None else opts.det_shape
             self.plan = None if opts is None else opts.plan
+            self.ws = None
+            if self.plan is not None:
+                n_view = self.n_ray // (self.det_shape[0] * self.det_shape[1])
+                self.ws = torch.empty(n_view, device=self.device)
         else:
             self.table = proj.tvals
             self.zero_rule = N.ZERO_EXACT if proj.semantics == "torch" else N.ZERO_THRESH
@@ -169,7 +173,7 @@ class ProjectorOps:
                                                out=out)
             if self.plan is not None:
                 return _C.back_project_3d_brick(y, self.src, self.dst, self.det_shape, self.plan, self.minv, self.b,
-                                                n_x, n_y, n_z, mode=self.mode, out=out)
+                                                n_x, n_y, n_z, mode=self.mode, out=out, workspace=self.ws)
             return _C.back_project_3d_on_the_fly_cuda(y, self.src, self.dst, self.minv, self.b, n_x, n_y, n_z,
                                                       mode=self.mode, det_shape=self.det_shape, out=out)
         n_row, n_col = self.shape

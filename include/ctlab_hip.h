@@ -197,7 +197,7 @@ int ctlab_cone_frames_3d(const float* src, const float* dst, int64_t n_view, int
 /* Brick plan of a geometry: for every 32x32x8 brick of the grid and every
  * view, the rectangle of detector pixels that can reach the brick (the
  * brick's shadow) and the running total of rectangle sizes; then the total
- * per brick and each view's midpoint error bound.
+ * per brick, each view's midpoint error bound and a copy of the frames.
  * ctlab_brick_plan_ints_3d returns the int32 size of the plan buffer. */
 int64_t ctlab_brick_plan_ints_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view);
 int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, int64_t det_cols, int64_t n_x,
@@ -213,11 +213,12 @@ int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t det_rows, i
  * still exact).  out [batch, n_x, n_y, n_z] is overwritten (or added to when
  * accumulate != 0).  Alternative schedule of the back projection that
  * replaces _C.back_project_3d_on_the_fly_cuda (bindings.cpp:101-103) for
- * rays in the [view][rows][cols] layout. */
+ * rays in the [view][rows][cols] layout.  view_max is caller workspace of
+ * batch * n_view floats (each view's largest |sino|, written by the call). */
 int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
                                 int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
-                                int accumulate, float* out, ctlab_stream_t stream);
+                                int accumulate, float* out, float* view_max, ctlab_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Many 2D scans in one persistent launch (no reference counterpart: the   */

@@ -51,3 +51,14 @@ def test_scan2d_record_layout():
     expect = {"image": 0, "sino": 8, "src": 16, "dst": 24, "minv": 32, "b": 48, "n_row": 56, "n_col": 60,
               "n_ray": 64, "mode": 72, "zero_rule": 76, "item_begin": 80}
     assert {k: SCAN_DTYPE.fields[k][1] for k in expect} == expect and SCAN_DTYPE.itemsize == 88
+
+
+def test_brick_plan_size():
+    """Plan = per brick and view 4 ints, per brick a total, per view E and a 16-float frame."""
+    from ct_laboratory_amd import _native as N
+
+    L = N.load()
+    for (nx, ny, nz, nv) in ((256, 256, 256, 720), (70, 45, 37, 11), (1, 1, 1, 1)):
+        nb = -(-nx // 32) * -(-ny // 32) * -(-nz // 8)
+        assert L.ctlab_brick_plan_ints_3d(nx, ny, nz, nv) == nb * (4 * nv + 1) + 17 * nv
+    assert L.ctlab_brick_plan_ints_3d(0, 4, 4, 1) == -1
