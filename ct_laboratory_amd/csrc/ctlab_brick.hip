@@ -441,6 +441,15 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
     }
 }
 
+// Round-half-even of c (|c| < 2^50) to a 64-bit two's-complement integer: c
+// is exact in double, and adding 1.5 * 2^52 lands in [2^52, 2^53), where the
+// double spacing is 1, so that addition rounds c half-to-even and the integer
+// sits in the low bits of the sum's representation.
+__device__ __forceinline__ unsigned long long to_fixed(float c) {
+    const double y = (double)c + 6755399441055744.0;  // 1.5 * 2^52
+    return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
+}
+
 // Longest segment any ray can have inside one voxel: the largest body
 // diagonal of the voxel parallelepiped, |M d| for d in {(1,1,1), (1,1,-1),
 // (1,-1,1), (-1,1,1)}, with M the inverse of M_inv.  Evaluated in double.
@@ -563,47 +572,58 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     // 2. power-of-two scale 2^F.  A ray's segments in one voxel add up to at
     // most the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the
     // factor 2 covers the rounding of len and of the product.  2^F maps that
-    // bound below 2^62 (no overflow of the signed 64-bit sums).  It is applied
-    // to each sinogram value as two float factors, so any F in [-250, 250] is
-    // exact.  The absolute error of a voxel is at most half a quantum 2^-F per
-    // contribution.  Non-finite data, or products that could overflow a float,
-    // take float atomics into global memory instead (INF/NaN propagate as with
-    // the reference's atomics).
+    // bound below 2^62 (no overflow of the signed 64-bit sums) and each single
+    // contribution below 2^50 (the range of the conversion in to_fixed).  It is
+    // applied to each sinogram value as two float factors, so any F in
+    // [-250, 250] is exact.  The absolute error of a voxel is at most half a
+    // quantum 2^-F per contribution.  Non-finite data, or products that could
+    // overflow a float, take float atomics into global memory instead (INF/NaN
+    // propagate as with the reference's atomics).
     const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
     const bool fixed = g.finite && cmax <= 1.0e38;
     const double bound = cmax * (double)(total > 0 ? total : 1);
     const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;  // bound < 2^e
-    const int F = fixed ? max(-250, min(62 - e, 250)) : 0;
+    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;   // cmax < 2^ec
+    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
     const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);  // 2^F as two normal floats
-    if (!fixed) {
-        for (int i = tid; i < nvox; i += kBrickThreads) {
-            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-            const int64_t gi =
-                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-            if (!accumulate) o[gi] = 0.0f;
-        }
-        __syncthreads();
-    }
 
     // 3. trace and accumulate
-    auto add = [&](uint32_t li, float c) {
-        if (fixed) {
-            atomicAdd(&acc[li], (unsigned long long)__float2ll_rn(c));
-        } else {
-            const uint32_t r = li % kStrideX;
-            const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
-            const int64_t gi =
-                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-            unsafeAtomicAdd(o + gi, c);
-        }
-    };
     if (vmax != 0.0f) {
-        if (g.diag)
-            brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1, s2,
-                                    queue, bcnt, add);
-        else
-            brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1, s2,
-                                     queue, bcnt, add);
+        if (fixed) {
+            auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
+            if (g.diag)
+                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                                        s2, queue, bcnt, add);
+            else
+                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                                         s2, queue, bcnt, add);
+        } else {
+            for (int i = tid; i < nvox; i += kBrickThreads) {
+                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                const int64_t gi =
+                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+                if (!accumulate) o[gi] = 0.0f;
+            }
+            __syncthreads();
+            auto add = [&](uint32_t li, float c) {
+                const uint32_t r = li % kStrideX;
+                const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
+                const int64_t gi =
+                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+                unsafeAtomicAdd(o + gi, c);
+            };
+            if (g.diag)
+                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                                        s2, queue, bcnt, add);
+            else
+                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                                         s2, queue, bcnt, add);
+        }
+    } else if (!fixed && !accumulate) {
+        for (int i = tid; i < nvox; i += kBrickThreads) {
+            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+            o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
+        }
     }
     if (!fixed) return;
     __syncthreads();

@@ -156,6 +156,14 @@ __host__ __device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint3
 #endif
 }
 
+// r * stride for an index r < 2^24 (the product is only used when r is in
+// range): a shift or constant multiply when the stride is known at compile
+// time, else the full-rate 24-bit multiply.
+__host__ __device__ __forceinline__ uint32_t mul_stride(uint32_t r, int stride) {
+    if (__builtin_constant_p(stride) && (stride & (stride - 1)) == 0) return r * (uint32_t)stride;
+    return mad24(r, (uint32_t)stride, 0u);
+}
+
 // Round half to even, range check and flat index ((i*n_y + j)*n_z + k as
 // i*(n_y*n_z) + (j*n_z + k)).  A NaN coordinate clamps to -1 (fmaxf returns the
 // non-NaN operand) and is out of range, as round(NaN).long() is in the
@@ -196,13 +204,13 @@ __host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w,
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const float f = c[a] + kMagic;
-        r[a] = (uint32_t)(__builtin_bit_cast(int32_t, f) - 0x4b400000 - w.lo[a]);
+        r[a] = __builtin_bit_cast(uint32_t, f) - (0x4b400000u + (uint32_t)w.lo[a]);
         ok = ok && (r[a] < (uint32_t)w.ext[a]);
     }
     if (ND == 3) {
-        *idx = mad24(r[0], (uint32_t)w.stride[0], mad24(r[1], (uint32_t)w.stride[1], r[ND - 1]));
+        *idx = mul_stride(r[0], w.stride[0]) + mul_stride(r[1], w.stride[1]) + r[ND - 1];
     } else {
-        *idx = mad24(r[0], (uint32_t)w.stride[0], r[ND - 1]);
+        *idx = mul_stride(r[0], w.stride[0]) + r[ND - 1];
     }
     return ok;
 }

@@ -241,8 +241,9 @@ def test_brick_back_projection(dev, mname, mode):
 def test_brick_signed_nonfinite_deterministic(dev, mname, mode):
     """Fixed-point brick accumulation: signed data (checked with the A|y| bound
     plus the fixed-point quantum: a voxel's absolute error is at most half a
-    quantum per contribution, the quantum being below 4 * max|y| * diag * rays /
-    2^62 of the brick, far under 1e-10 * max|y| here), bit-identical repeats,
+    quantum per contribution, the quantum being below the larger of
+    4 * max|y| * diag * rays / 2^62 and 4 * max|y| * diag / 2^50 of the brick,
+    far under 1e-10 * max|y| here), bit-identical repeats,
     value scales near both ends of the float range, and the float fallback for
     bricks whose rays carry INF/NaN."""
     shape = (40, 33, 36)
