@@ -262,12 +262,17 @@ __host__ __device__ __forceinline__ int first_plane(float s, float d, float y, i
     if (step > 0) {
         const float f = ceilf(c + 0.5f);
         p = (f >= (float)plo) ? ((f <= (float)(phi + 1)) ? (int)f : phi + 1) : plo;
+        // The guess is off by at most a step or two: no unrolling.
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
         while (p > plo && crossing_p<MODE>(p - 1, s, d, y) >= t_lo) --p;
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
         while (p <= phi && crossing_p<MODE>(p, s, d, y) < t_lo) ++p;
     } else {
         const float f = floorf(c + 0.5f);
         p = (f >= (float)(plo - 1)) ? ((f <= (float)phi) ? (int)f : phi) : plo - 1;
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
         while (p < phi && crossing_p<MODE>(p + 1, s, d, y) >= t_lo) ++p;
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
         while (p >= plo && crossing_p<MODE>(p, s, d, y) < t_lo) --p;
     }
     return p;
