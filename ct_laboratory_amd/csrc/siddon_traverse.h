@@ -115,6 +115,21 @@ __host__ __device__ __forceinline__ void to_voxel(const Grid<ND>& g, const float
     }
 }
 
+// to_voxel for a diagonal M_inv and a finite point: each off-diagonal term of
+// the chain is a signed zero, and adding it changes at most the sign of a zero
+// result.  No use of a voxel coordinate depends on the sign of a zero (it
+// enters only as s in RN(f - s) with f = p - 0.5 != 0, in differences, and in
+// magnitudes), so the traversal is the same.
+template <int ND, int MODE, bool DIAG>
+__host__ __device__ __forceinline__ void to_voxel_t(const Grid<ND>& g, const float* p, float* out) {
+    if (DIAG) {
+#pragma unroll
+        for (int a = 0; a < ND; ++a) out[a] = g.minv[a * ND + a] * (p[a] - g.b[a]);
+    } else {
+        to_voxel<ND, MODE>(g, p, out);
+    }
+}
+
 // Midpoint voxel coordinates from the world midpoint (3d_torch.py:244-250;
 // 3d.cu:985-991).
 template <int ND, int MODE, bool DIAG>
@@ -280,30 +295,39 @@ __host__ __device__ __forceinline__ int first_plane(float s, float d, float y, i
 
 // Merge of the per-axis crossing sequences, held in scalar registers (no
 // arrays, so nothing is dynamically indexed).  Axis a keeps the plane
-// coordinate f_a = p_a - 0.5 of its pending crossing, its step (+-1, 0 when
-// inactive), e_a = the plane coordinate one past its last plane, and the
-// pending crossing value n_a (INF when exhausted or past t = 1).
+// coordinate f_a = p_a - 0.5 of its pending crossing, e_a = the plane
+// coordinate one past its last plane, and the pending crossing value n_a (INF
+// when exhausted or past t = 1).  An axis traversed downwards is stored
+// mirrored (s, d, y, f and e negated): every crossing value is unchanged, since
+// RN(-x) = -RN(x) and the fma forms see the same products, so each axis steps
+// by +1 and needs no step register.
 template <int ND, int MODE>
 struct Merge {
     float s0 = 0, s1 = 0, s2 = 0, d0 = 1, d1 = 1, d2 = 1, y0 = 1, y1 = 1, y2 = 1;
-    float f0 = 0, f1 = 0, f2 = 0, st0 = 0, st1 = 0, st2 = 0, e0 = 0, e1 = 0, e2 = 0;
+    float f0 = 0, f1 = 0, f2 = 0, e0 = 0, e1 = 0, e2 = 0;
     float n0 = __builtin_huge_valf(), n1 = __builtin_huge_valf(), n2 = __builtin_huge_valf();
 
     // Initialises axis a (a is a compile-time constant after unrolling).
     __host__ __device__ __forceinline__ void set(int a, float s, float d, float y, int p, int step, int plo,
                                                  int phi, float t_next) {
-        const float f = (float)p - 0.5f;
-        const float e = (step > 0) ? (float)phi + 0.5f : (float)plo - 1.5f;
-        const float st = (float)step;
-        if (a == 0) { s0 = s; d0 = d; y0 = y; f0 = f; st0 = st; e0 = e; n0 = t_next; }
-        if (a == 1) { s1 = s; d1 = d; y1 = y; f1 = f; st1 = st; e1 = e; n1 = t_next; }
-        if (a == 2) { s2 = s; d2 = d; y2 = y; f2 = f; st2 = st; e2 = e; n2 = t_next; }
+        float f = (float)p - 0.5f;
+        float e = (step > 0) ? (float)phi + 0.5f : (float)plo - 1.5f;
+        if (step < 0) {
+            s = -s;
+            d = -d;
+            y = -y;
+            f = -f;
+            e = -e;
+        }
+        if (a == 0) { s0 = s; d0 = d; y0 = y; f0 = f; e0 = e; n0 = t_next; }
+        if (a == 1) { s1 = s; d1 = d; y1 = y; f1 = f; e1 = e; n1 = t_next; }
+        if (a == 2) { s2 = s; d2 = d; y2 = y; f2 = f; e2 = e; n2 = t_next; }
     }
 
     // Advanced candidate of one axis: the crossing after its pending one.
-    __host__ __device__ __forceinline__ static float advance(float f, float st, float s, float d, float y, float e,
+    __host__ __device__ __forceinline__ static float advance(float f, float s, float d, float y, float e,
                                                              float* f_out) {
-        const float fn = f + st;
+        const float fn = f + 1.0f;
         const float tn = crossing_f<MODE>(fn, s, d, y);
         *f_out = fn;
         // Past the grid's last plane the axis is exhausted.  Values above 1 need
@@ -321,13 +345,13 @@ struct Merge {
         const bool k0 = (n0 == m);
         const bool k1 = !k0 && (n1 == m);
         float g0, g1, g2 = 0.0f;
-        const float c0 = advance(f0, st0, s0, d0, y0, e0, &g0);
-        const float c1 = advance(f1, st1, s1, d1, y1, e1, &g1);
+        const float c0 = advance(f0, s0, d0, y0, e0, &g0);
+        const float c1 = advance(f1, s1, d1, y1, e1, &g1);
         f0 = k0 ? g0 : f0;
         n0 = k0 ? c0 : n0;
         if (ND == 3) {
             const bool k2 = !k0 && !k1;
-            const float c2 = advance(f2, st2, s2, d2, y2, e2, &g2);
+            const float c2 = advance(f2, s2, d2, y2, e2, &g2);
             f1 = k1 ? g1 : f1;
             n1 = k1 ? c1 : n1;
             f2 = k2 ? g2 : f2;
@@ -409,8 +433,8 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         if (rl < 1.0e-12f) return;
     }
 
-    to_voxel<ND, MODE>(g, src, sv);
-    to_voxel<ND, MODE>(g, dst, dv);
+    to_voxel_t<ND, MODE, DIAG>(g, src, sv);
+    to_voxel_t<ND, MODE, DIAG>(g, dst, dv);
 
     float dir[ND], y[ND];
     int step[ND];
