@@ -271,39 +271,6 @@ __global__ __launch_bounds__(256) void k_brick_plan(Dims3 dims, BrickGrid bg, in
     if (tid == 0) totals[bid] = running;
 }
 
-// Walks the items of a brick in the order one thread visits them (item,
-// item + kBrickThreads, ...): the view of an item never decreases, so after a
-// binary search for the first item the view advances linearly, usually by zero
-// or one step.  The host guarantees total + kBrickThreads < 2^32, so the 32-bit
-// item counter cannot wrap.
-struct BrickItems {
-    const int32_t* __restrict__ p;
-    int n_view, rows, cols;
-    int v, r, c;  // view, detector row and column of the last item
-
-    __device__ __forceinline__ BrickItems(const int32_t* __restrict__ p_, int n_view_, int rows_, int cols_,
-                                          uint32_t first)
-        : p(p_), n_view(n_view_), rows(rows_), cols(cols_) {
-        int lo = 0, hi = n_view;  // largest view whose prefix <= first
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if ((uint32_t)p[mid * kPlanInts + 3] <= first) lo = mid; else hi = mid;
-        }
-        v = lo;
-    }
-
-    // Ray of an item not below any item passed before.
-    __device__ __forceinline__ int64_t ray(uint32_t item) {
-        while (v + 1 < n_view && (uint32_t)p[(v + 1) * kPlanInts + 3] <= item) ++v;
-        const int32_t* e = p + v * kPlanInts;
-        const uint32_t local = item - (uint32_t)e[3];
-        const uint32_t w = (uint32_t)e[2];
-        r = e[0] + (int)(local / w);
-        c = e[1] + (int)(local % w);
-        return ((int64_t)v * rows + r) * cols + c;
-    }
-};
-
 // Upper estimate of the number of plane crossings of a ray inside the box
 // [blo, bhi] (voxel coordinates, the brick's cells widened by at least the
 // midpoint error bound), 0 when the ray misses the box on t in [0, 1].  Any
@@ -357,24 +324,51 @@ __device__ __forceinline__ uint32_t frame_estimate(const float* __restrict__ f, 
     return slab_estimate(sv, dv, blo, bhi, f[F_DEVS] + f[F_DEVD]);
 }
 
-// Items are traced in chunks of kChunk.  Per chunk every thread takes two
-// items, drops rays with a zero value or no crossing of the brick, and files
-// the rest into buckets by their estimated crossing count (powers of two);
-// then the chunk's rays are traced in bucket order, so the 64 lanes of a wave
-// trace rays of similar length instead of idling behind the longest one.
+// The hardware deals consecutive workgroups round-robin to the 8 XCDs.  Block
+// b takes brick j of XCD b % 8's contiguous run of bricks, so the bricks that
+// run at the same time on one XCD are neighbours (z fastest, then y) and the
+// rays they share stay in that XCD's L2.
+__device__ __forceinline__ uint32_t brick_of_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u, j = b / 8u;
+    return (x < r) ? x * (q + 1u) + j : r * (q + 1u) + (x - r) * q + j;
+}
+
+// Items are traced in chunks of at most kChunk.  Per chunk the plan entries
+// and frames of the next kStage views are staged in LDS (one load latency for
+// the block instead of a dependent chain per item); every thread then takes
+// two items, finds their views by a binary search in LDS, drops rays with a
+// zero value or no crossing of the brick, and files the rest into buckets by
+// their estimated crossing count (powers of two); then the chunk's rays are
+// traced in bucket order, so the 64 lanes of a wave trace rays of similar
+// length instead of idling behind the longest one.  A chunk ends at the first
+// item beyond the staged views.
 constexpr int kChunk = 2 * kBrickThreads;
 constexpr int kBuckets = 8;
+constexpr int kStage = 64;
+
+// Largest j in [0, n) with prefix(j) <= item, for prefixes non-decreasing in j
+// and prefix(0) <= item.
+__device__ __forceinline__ int staged_view(const int4* s_plan, int n, uint32_t item) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((uint32_t)s_plan[mid].w <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 // Traces every ray of the brick and hands (window index, contribution) to
 // add().  Each sinogram value is first multiplied by s1*s2 (powers of two, so
 // exactly, and the product val*len scales with it).  e_brick bounds the
 // midpoint error of every ray of the brick (non-finite or above 8: unknown).
+// s_plan holds kStage + 1 plan entries and s_frame kStage frames (LDS).
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
                                             const int32_t* __restrict__ p, const float* __restrict__ frames,
                                             int n_view, int rows, int cols, uint32_t total, float e_brick, float s1,
-                                            float s2, uint32_t* queue, uint32_t* bcnt, Add&& add) {
+                                            float s2, uint32_t* queue, uint32_t* bcnt, int4* s_plan, float* s_frame,
+                                            Add&& add) {
     const int tid = threadIdx.x;
     const bool e_ok = e_brick <= 8.0f;
     const float e_hint = e_ok ? e_brick : -1.0f;
@@ -385,23 +379,50 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
         blo[a] = (float)win.lo[a] - 0.5f - m;
         bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
     }
-    BrickItems it(p, n_view, rows, cols, tid);
-    for (uint32_t c0 = 0; c0 < total; c0 += kChunk) {
+    // view of item 0: the largest view whose prefix is 0 (block-uniform)
+    int vs = 0;
+    {
+        int lo = 0, hi = n_view;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)p[mid * kPlanInts + 3] == 0u) lo = mid; else hi = mid;
+        }
+        vs = lo;
+    }
+    uint32_t c0 = 0;
+    while (c0 < total) {
         if (tid <= kBuckets) bcnt[tid] = 0u;
+        if (tid <= kStage) {
+            const int v = vs + tid;
+            int4 e = make_int4(0, 0, 1, (int)total);  // past the last view: prefix = total
+            if (v < n_view) e = *reinterpret_cast<const int4*>(p + (int64_t)v * kPlanInts);
+            s_plan[tid] = e;
+            if (tid < kStage && v < n_view) {
+                const float* __restrict__ f = frames + (int64_t)v * kFrameFloats;
+#pragma unroll
+                for (int k = 0; k < kFrameFloats; ++k) s_frame[tid * kFrameFloats + k] = f[k];
+            }
+        }
         __syncthreads();
+        const uint32_t lim = (uint32_t)s_plan[kStage].w;  // first item beyond the staged views
+        const uint32_t c1 = min(c0 + (uint32_t)kChunk, min(lim, total));
         uint32_t rk[2], key[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             key[k] = ~0u;
             rk[k] = 0u;
             const uint32_t item = c0 + (uint32_t)tid + (uint32_t)(k * kBrickThreads);
-            if (item < total) {
-                const uint32_t ray = (uint32_t)it.ray(item);
+            if (item < c1) {
+                const int j = staged_view(s_plan, kStage, item);
+                const int4 e = s_plan[j];
+                const uint32_t local = item - (uint32_t)e.w, w = (uint32_t)e.z;
+                const int r = e.x + (int)(local / w), c = e.y + (int)(local % w);
+                const uint32_t ray = (uint32_t)(((int64_t)(vs + j) * rows + r) * cols + c);
                 rk[k] = ray;
                 if (sn[ray] != 0.0f) {
-                    const float* __restrict__ f = frames + (int64_t)it.v * kFrameFloats;
+                    const float* f = s_frame + j * kFrameFloats;
                     const uint32_t est = (f[F_VALID] != 0.0f)
-                                             ? frame_estimate(f, it.r, it.c, blo, bhi)
+                                             ? frame_estimate(f, r, c, blo, bhi)
                                              : segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
                     if (est) {
                         const uint32_t bk = min(32u - (uint32_t)__clz(est - 1u), (uint32_t)(kBuckets - 1));
@@ -437,6 +458,9 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
             trace_ray_window<3, MODE, DIAG>(
                 g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
         }
+        // the next chunk starts in the largest staged view whose prefix <= c1
+        vs += staged_view(s_plan, kStage + 1, c1);
+        c0 = c1;
         __syncthreads();
     }
 }
@@ -514,8 +538,10 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ uint32_t queue[kChunk];
     __shared__ uint32_t bcnt[kBuckets + 1];
     __shared__ float s_red[kBrickThreads / 64];
+    __shared__ int4 s_plan[kStage + 1];
+    __shared__ float s_frame[kStage * kFrameFloats];
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
+    const int bid = (int)brick_of_block(blockIdx.x, gridDim.x);
     const Window<3> win = brick_window(dims, bg, bid);
     for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
 
@@ -593,10 +619,10 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
             auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
             if (g.diag)
                 brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
-                                        s2, queue, bcnt, add);
+                                        s2, queue, bcnt, s_plan, s_frame, add);
             else
                 brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
-                                         s2, queue, bcnt, add);
+                                         s2, queue, bcnt, s_plan, s_frame, add);
         } else {
             for (int i = tid; i < nvox; i += kBrickThreads) {
                 const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
@@ -614,10 +640,10 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
             };
             if (g.diag)
                 brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
-                                        s2, queue, bcnt, add);
+                                        s2, queue, bcnt, s_plan, s_frame, add);
             else
                 brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
-                                         s2, queue, bcnt, add);
+                                         s2, queue, bcnt, s_plan, s_frame, add);
         }
     } else if (!fixed && !accumulate) {
         for (int i = tid; i < nvox; i += kBrickThreads) {

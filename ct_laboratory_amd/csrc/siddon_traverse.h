@@ -365,6 +365,42 @@ struct Merge {
     }
 };
 
+// 0.5f * x and 2.0f * x (round to nearest even, denormals and INF/NaN
+// included) in integer operations on the bit pattern.  For wave-uniform x they
+// run on the scalar unit and stay in scalar registers, where the float
+// products would occupy (or be recomputed in) vector registers: gfx950 has no
+// scalar float arithmetic.
+__host__ __device__ __forceinline__ float half_exact(float x) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const uint32_t sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    uint32_t r;
+    if (mag >= 0x7f800000u) {
+        r = u;  // INF, NaN
+    } else if (mag >= 0x01000000u) {
+        r = u - 0x00800000u;  // exponent >= 2: the result is normal and exact
+    } else {
+        // exponent 0 or 1: value = mag * 2^-149, so halve mag with ties to even
+        r = sign | ((mag >> 1) + (mag & (mag >> 1) & 1u));
+    }
+    return __builtin_bit_cast(float, r);
+}
+
+__host__ __device__ __forceinline__ float twice_exact(float x) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const uint32_t sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    uint32_t r;
+    if (mag >= 0x7f800000u) {
+        r = u;  // INF, NaN
+    } else if (mag >= 0x7f000000u) {
+        r = sign | 0x7f800000u;  // exponent 254: overflows to INF
+    } else if (mag >= 0x00800000u) {
+        r = u + 0x00800000u;  // normal
+    } else {
+        r = sign | (mag << 1);  // zero or denormal: exact, possibly normal
+    }
+    return __builtin_bit_cast(float, r);
+}
+
 // Midpoint voxel of a segment from its end points, folded: with S = P0+P1,
 // RN(Minv*RN(RN(0.5*S) - b)) == RN((0.5*Minv) * RN(S - 2b)) exactly (scaling
 // by a power of two commutes with rounding), so h = 0.5*Minv and b2 = 2b are
@@ -376,9 +412,9 @@ struct MidIndex {
     float b2[ND];
     __host__ __device__ __forceinline__ void init(const Grid<ND>& g) {
 #pragma unroll
-        for (int i = 0; i < ND * ND; ++i) h[i] = 0.5f * g.minv[i];
+        for (int i = 0; i < ND * ND; ++i) h[i] = half_exact(g.minv[i]);
 #pragma unroll
-        for (int a = 0; a < ND; ++a) b2[a] = 2.0f * g.b[a];
+        for (int a = 0; a < ND; ++a) b2[a] = twice_exact(g.b[a]);
     }
     __host__ __device__ __forceinline__ void from_ends(const float* p0, const float* p1, float* c) const {
         float sh[ND];

@@ -1,5 +1,7 @@
 // Test-only: runs the product's traversal (siddon_traverse.h) on the CPU so its
 // segment lists can be compared with the oracle without a GPU.
+#include <string.h>
+
 #include "../../ct_laboratory_amd/csrc/siddon_traverse.h"
 
 using namespace ctlab;
@@ -97,4 +99,29 @@ extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const in
         }
     }
     return 0;
+}
+
+// Every float bit pattern in [lo, hi): counts where half_exact / twice_exact
+// differ in bits from the host's 0.5f * x and 2.0f * x (NaNs compare as NaN).
+extern "C" int64_t host_scale_mismatches(uint64_t lo, uint64_t hi) {
+    int64_t bad = 0;
+    for (uint64_t k = lo; k < hi; ++k) {
+        const uint32_t u = (uint32_t)k;
+        float x;
+        ::memcpy(&x, &u, 4);
+        const float h = ctlab::half_exact(x), t = ctlab::twice_exact(x);
+        volatile float hx = 0.5f * x, tx = 2.0f * x;
+        const float hr = hx, tr = tx;
+        uint32_t a, b, c, d;
+        ::memcpy(&a, &h, 4);
+        ::memcpy(&b, &hr, 4);
+        ::memcpy(&c, &t, 4);
+        ::memcpy(&d, &tr, 4);
+        if (x != x) {
+            bad += !(h != h) + !(t != t);
+        } else {
+            bad += (a != b) + (c != d);
+        }
+    }
+    return bad;
 }

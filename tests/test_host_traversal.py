@@ -137,3 +137,13 @@ def test_windows_partition_the_traversal(host, host_window, mode, nd):
             a = sorted(zip(full[r][0].tolist(), full[r][1].tolist()))
             bb = sorted(zip(got[r][0], got[r][1]))
             assert a == bb, (trial, r)
+
+
+def test_half_and_twice_exact_on_every_float():
+    """The traversal's integer forms of 0.5*x and 2*x (used for its midpoint
+    constants) equal the FPU products bit for bit on all 2^32 patterns."""
+    assert os.path.exists(LIB), "run __graft_entry__.build() first"
+    L = ctypes.CDLL(LIB)
+    L.host_scale_mismatches.restype = ctypes.c_int64
+    L.host_scale_mismatches.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    assert L.host_scale_mismatches(0, 1 << 32) == 0
