@@ -324,15 +324,6 @@ __device__ __forceinline__ uint32_t frame_estimate(const float* __restrict__ f, 
     return slab_estimate(sv, dv, blo, bhi, f[F_DEVS] + f[F_DEVD]);
 }
 
-// The hardware deals consecutive workgroups round-robin to the 8 XCDs.  Block
-// b takes brick j of XCD b % 8's contiguous run of bricks, so the bricks that
-// run at the same time on one XCD are neighbours (z fastest, then y) and the
-// rays they share stay in that XCD's L2.
-__device__ __forceinline__ uint32_t brick_of_block(uint32_t b, uint32_t nb) {
-    const uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u, j = b / 8u;
-    return (x < r) ? x * (q + 1u) + j : r * (q + 1u) + (x - r) * q + j;
-}
-
 // Items are traced in chunks of at most kChunk.  Per chunk the plan entries
 // and frames of the next kStage views are staged in LDS (one load latency for
 // the block instead of a dependent chain per item); every thread then takes
@@ -366,7 +357,8 @@ template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
                                             const int32_t* __restrict__ p, const float* __restrict__ frames,
-                                            int n_view, int rows, int cols, uint32_t total, float e_brick, float s1,
+                                            int n_view, int rows, int cols, uint32_t item_lo, uint32_t item_hi,
+                                            uint32_t total, float e_brick, float s1,
                                             float s2, uint32_t* queue, uint32_t* bcnt, int4* s_plan, float* s_frame,
                                             Add&& add) {
     const int tid = threadIdx.x;
@@ -379,18 +371,18 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
         blo[a] = (float)win.lo[a] - 0.5f - m;
         bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
     }
-    // view of item 0: the largest view whose prefix is 0 (block-uniform)
+    // view of item_lo: the largest view whose prefix is <= item_lo (block-uniform)
     int vs = 0;
     {
         int lo = 0, hi = n_view;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if ((uint32_t)p[mid * kPlanInts + 3] == 0u) lo = mid; else hi = mid;
+            if ((uint32_t)p[mid * kPlanInts + 3] <= item_lo) lo = mid; else hi = mid;
         }
         vs = lo;
     }
-    uint32_t c0 = 0;
-    while (c0 < total) {
+    uint32_t c0 = item_lo;
+    while (c0 < item_hi) {
         if (tid <= kBuckets) bcnt[tid] = 0u;
         if (tid <= kStage) {
             const int v = vs + tid;
@@ -405,7 +397,7 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
         }
         __syncthreads();
         const uint32_t lim = (uint32_t)s_plan[kStage].w;  // first item beyond the staged views
-        const uint32_t c1 = min(c0 + (uint32_t)kChunk, min(lim, total));
+        const uint32_t c1 = min(c0 + (uint32_t)kChunk, min(lim, item_hi));
         uint32_t rk[2], key[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -474,6 +466,18 @@ __device__ __forceinline__ unsigned long long to_fixed(float c) {
     return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
 }
 
+// *p += v with the shader's IEEE single addition (compare-and-swap loop; each
+// failed swap means another writer's add landed, so the loop ends).
+__device__ __forceinline__ void add_f32_ieee(float* p, float v) {
+    unsigned int* q = reinterpret_cast<unsigned int*>(p);
+    unsigned int old = *q;
+    while (true) {
+        const unsigned int seen = atomicCAS(q, old, __float_as_uint(__uint_as_float(old) + v));
+        if (seen == old) break;
+        old = seen;
+    }
+}
+
 // Longest segment any ray can have inside one voxel: the largest body
 // diagonal of the voxel parallelepiped, |M d| for d in {(1,1,1), (1,1,-1),
 // (1,-1,1), (-1,1,1)}, with M the inverse of M_inv.  Evaluated in double.
@@ -526,30 +530,28 @@ __global__ __launch_bounds__(256) void k_view_absmax(const float* __restrict__ s
         view_max[(int64_t)blockIdx.y * n_view + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
+// One work unit: part `part` of brick `bid` of batch element bi.  Part k of
+// split takes the items of views [k * n_view / split, (k + 1) * n_view / split).
 template <int MODE>
-__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick(
+__device__ __forceinline__ void brick_unit(
     const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
     BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
     const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ frames,
     const float* __restrict__ src,
     const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b,
-    const float* __restrict__ view_max, int accumulate) {
-    __shared__ unsigned long long acc[kBrickLds];
-    __shared__ uint32_t queue[kChunk];
-    __shared__ uint32_t bcnt[kBuckets + 1];
-    __shared__ float s_red[kBrickThreads / 64];
-    __shared__ int4 s_plan[kStage + 1];
-    __shared__ float s_frame[kStage * kFrameFloats];
+    const float* __restrict__ view_max, int accumulate, int split, int bid, int part, int64_t bi,
+    unsigned long long* acc, uint32_t* queue, uint32_t* bcnt, float* s_red, int4* s_plan, float* s_frame) {
     const int tid = threadIdx.x;
-    const int bid = (int)brick_of_block(blockIdx.x, gridDim.x);
     const Window<3> win = brick_window(dims, bg, bid);
     for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
 
     const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
-    const int64_t bi = blockIdx.y;
     const float* __restrict__ sn = sino + bi * n_ray;
     const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
     const uint32_t total = totals[bid];
+    const int v_lo = (int)((int64_t)part * n_view / split), v_hi = (int)((int64_t)(part + 1) * n_view / split);
+    const uint32_t item_lo = (v_lo < n_view) ? (uint32_t)p[(int64_t)v_lo * kPlanInts + 3] : total;
+    const uint32_t item_hi = (v_hi < n_view) ? (uint32_t)p[(int64_t)v_hi * kPlanInts + 3] : total;
     float* __restrict__ o = out + bi * vol_elems;
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
@@ -618,17 +620,18 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
         if (fixed) {
             auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
             if (g.diag)
-                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, item_lo, item_hi, total, e_brick, s1,
                                         s2, queue, bcnt, s_plan, s_frame, add);
             else
-                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, item_lo, item_hi, total, e_brick, s1,
                                          s2, queue, bcnt, s_plan, s_frame, add);
         } else {
-            for (int i = tid; i < nvox; i += kBrickThreads) {
-                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-                const int64_t gi =
-                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-                if (!accumulate) o[gi] = 0.0f;
+            if (split == 1 && !accumulate) {  // split > 1: the host zeroed out
+                for (int i = tid; i < nvox; i += kBrickThreads) {
+                    const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                    o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] =
+                        0.0f;
+                }
             }
             __syncthreads();
             auto add = [&](uint32_t li, float c) {
@@ -639,29 +642,59 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
                 unsafeAtomicAdd(o + gi, c);
             };
             if (g.diag)
-                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, item_lo, item_hi, total, e_brick, s1,
                                         s2, queue, bcnt, s_plan, s_frame, add);
             else
-                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, total, e_brick, s1,
+                brick_items<MODE, false>(g, win, sn, src, dst, p, frames, n_view, rows, cols, item_lo, item_hi, total, e_brick, s1,
                                          s2, queue, bcnt, s_plan, s_frame, add);
         }
-    } else if (!fixed && !accumulate) {
+    } else if (!fixed && !accumulate && split == 1) {
         for (int i = tid; i < nvox; i += kBrickThreads) {
             const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
             o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
         }
     }
-    if (!fixed) return;
+    if (!fixed || (split > 1 && vmax == 0.0f)) return;
     __syncthreads();
 
-    // 4. write the brick (k fastest: coalesced rows of the volume)
+    // 4. write the brick (k fastest: coalesced rows of the volume).  With two
+    // parts each adds its rounded sum to the zeroed output: 0 + a + b rounds
+    // the same in either order, so the result does not depend on which part
+    // finishes first.
     const double inv = ldexp(1.0, -F);
     for (int i = tid; i < nvox; i += kBrickThreads) {
         const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
         const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
         const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
-        o[gi] = accumulate ? o[gi] + v : v;
+        if (split > 1)
+            add_f32_ieee(o + gi, v);
+        else
+            o[gi] = accumulate ? o[gi] + v : v;
     }
+}
+
+// One workgroup per work unit (x) and batch element (y).  Consecutive units go
+// round-robin to the 8 XCDs, which keeps the XCDs' loads even (a contiguous
+// run of bricks per XCD would give the XCDs holding the centre of the volume
+// the most work).
+template <int MODE>
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    BrickGrid bg, int n_view, int rows, int cols, const int32_t* __restrict__ plan,
+    const uint32_t* __restrict__ totals, const float* __restrict__ view_e, const float* __restrict__ frames,
+    const float* __restrict__ src,
+    const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b,
+    const float* __restrict__ view_max, int accumulate, int split) {
+    __shared__ unsigned long long acc[kBrickLds];
+    __shared__ uint32_t queue[kChunk];
+    __shared__ uint32_t bcnt[kBuckets + 1];
+    __shared__ float s_red[kBrickThreads / 64];
+    __shared__ int4 s_plan[kStage + 1];
+    __shared__ float s_frame[kStage * kFrameFloats];
+    const uint32_t unit = blockIdx.x;
+    brick_unit<MODE>(sino, out, n_ray, vol_elems, dims, bg, n_view, rows, cols, plan, totals, view_e, frames, src, dst,
+                     minv, b, view_max, accumulate, split, (int)(unit / (uint32_t)split),
+                     (int)(unit % (uint32_t)split), (int64_t)blockIdx.y, acc, queue, bcnt, s_red, s_plan, s_frame);
 }
 
 }  // namespace ctlab
@@ -748,18 +781,26 @@ extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, con
     const float* view_e = reinterpret_cast<const float*>(plan + nb * (n_view * kPlanInts + 1));
     const float* frames = view_e + n_view;
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
-    dim3 grid((unsigned)nb, (unsigned)batch);
+    // Two work units per brick (halves of the views) halve the longest unit and
+    // with it the tail of the launch; they add into a zeroed output, which the
+    // accumulate mode cannot use.
+    const int split = accumulate ? 1 : 2;
+    dim3 grid((unsigned)(nb * split), (unsigned)batch);
     hipLaunchKernelGGL(k_view_absmax, dim3((unsigned)n_view, (unsigned)batch), dim3(256), 0, as_stream(stream), sino,
                        det_rows * det_cols, (int)n_view, view_max);
     st = check_launch(fn);
     if (st) return st;
+    if (split > 1) {
+        st = fill_f32(out, batch * V, 0.0f, as_stream(stream));
+        if (st) return st;
+    }
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL(k_back_brick<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
                            n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, frames, src,
-                           dst, M_inv, b, view_max, accumulate);
+                           dst, M_inv, b, view_max, accumulate, split);
     else
         hipLaunchKernelGGL(k_back_brick<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
                            n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, frames, src,
-                           dst, M_inv, b, view_max, accumulate);
+                           dst, M_inv, b, view_max, accumulate, split);
     return check_launch(fn);
 }
