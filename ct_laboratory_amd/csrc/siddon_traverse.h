@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace ctlab {
@@ -325,13 +327,16 @@ struct Merge {
     }
 
     // Advanced candidate of one axis: the crossing after its pending one.
+    // EXH: past its last plane the axis is exhausted (INF).  Without EXH the
+    // caller stops before any crossing beyond the last plane matters.  Values
+    // above 1 need no test: pop() only accepts crossings <= t_hi <= 1.
+    template <bool EXH>
     __host__ __device__ __forceinline__ static float advance(float f, float s, float d, float y, float e,
                                                              float* f_out) {
         const float fn = f + 1.0f;
         const float tn = crossing_f<MODE>(fn, s, d, y);
         *f_out = fn;
-        // Past the grid's last plane the axis is exhausted.  Values above 1 need
-        // no test: pop() only accepts crossings <= t_hi <= 1.
+        if (!EXH) return tn;
         return (fn == e) ? __builtin_huge_valf() : tn;
     }
 
@@ -339,19 +344,20 @@ struct Merge {
     // evaluates its own candidate from its own registers and only the results
     // are selected: selecting the operands instead lets the compiler turn the
     // per-axis state into an indexed stack array (scratch memory).
+    template <bool EXH = true>
     __host__ __device__ __forceinline__ bool pop(float t_hi, float& t) {
         const float m = (ND == 3) ? fminf(fminf(n0, n1), n2) : fminf(n0, n1);
         if (!(m <= t_hi)) return false;
         const bool k0 = (n0 == m);
         const bool k1 = !k0 && (n1 == m);
         float g0, g1, g2 = 0.0f;
-        const float c0 = advance(f0, s0, d0, y0, e0, &g0);
-        const float c1 = advance(f1, s1, d1, y1, e1, &g1);
+        const float c0 = advance<EXH>(f0, s0, d0, y0, e0, &g0);
+        const float c1 = advance<EXH>(f1, s1, d1, y1, e1, &g1);
         f0 = k0 ? g0 : f0;
         n0 = k0 ? c0 : n0;
         if (ND == 3) {
             const bool k2 = !k0 && !k1;
-            const float c2 = advance(f2, s2, d2, y2, e2, &g2);
+            const float c2 = advance<EXH>(f2, s2, d2, y2, e2, &g2);
             f1 = k1 ? g1 : f1;
             n1 = k1 ? c1 : n1;
             f2 = k2 ? g2 : f2;
@@ -476,6 +482,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
     int step[ND];
     float t_lo = 0.0f, t_hi = 1.0f;
     const float kInf = __builtin_huge_valf();
+    float e_max = 0.0f, t_cap = kInf;  // t_cap: first crossing of a plane beyond the window
 
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
@@ -493,6 +500,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         // are far below the 2^22 bound of flat_voxel_bounded.  Endpoints that far
         // out (~1e11 voxels) carry no usable float32 crossings anyway.
         if (!(E <= 8.0f)) return;
+        e_max = fmaxf(e_max, E);
         bool active;
         if (MODE == MODE_OTF) {
             active = fabsf(dir[a]) > 1.0e-12f;
@@ -518,6 +526,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         const float tau = 4.0f * E * fabsf(y[a]) + 9.5367431640625e-07f;  // + 2^-20
         t_lo = fmaxf(t_lo, t_in - tau);
         t_hi = fminf(t_hi, t_out + tau);
+        t_cap = fminf(t_cap, crossing_p<MODE>(up ? phi + 1 : plo - 1, sv[a], dir[a], y[a]));
     }
     if (!(t_lo <= t_hi)) return;
 
@@ -536,61 +545,76 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         mg.set(a, sv[a], dir[a], y[a], p, step[a], plo, phi, t);
     }
 
-    float t0;
-    if (MODE == MODE_OTF && t_lo == 0.0f) {
-        t0 = 0.0f;  // endpoint inserted by collect_hits (3d.cu:890)
-    } else if (!mg.pop(t_hi, t0)) {
-        return;
-    }
-    const bool close_at_one = (MODE == MODE_OTF) && (t_hi == 1.0f);
-
+    // Two forms of the segment loop.  With every midpoint error bound below
+    // one half (all practical geometries) the walk stops at the first crossing
+    // of a plane beyond the window, t_cap, and needs no per-axis exhaustion
+    // test: a crossing of plane phi + 1 is a reference crossing when the
+    // window ends inside the grid, and otherwise (a plane past the grid) it
+    // ends only a segment at least half a voxel beyond the grid, which no
+    // rounding within E < 1/2 brings back.  Segments past that plane are a
+    // voxel or more outside the window.  Larger bounds take the exhaustive
+    // form.
     MidIndex<ND, DIAG> mi;
     mi.init(g);
-    float p0[ND];
+    auto walk = [&](auto exh, float t_stop) {
+        constexpr bool EXH = decltype(exh)::value;
+        float t0;
+        if (MODE == MODE_OTF && t_lo == 0.0f) {
+            t0 = 0.0f;  // endpoint inserted by collect_hits (3d.cu:890)
+        } else if (!mg.template pop<EXH>(t_stop, t0)) {
+            return;
+        }
+        const bool close_at_one = (MODE == MODE_OTF) && (t_stop == 1.0f);
+        float p0[ND];
 #pragma unroll
-    for (int a = 0; a < ND; ++a) p0[a] = src[a] + t0 * vw[a];
+        for (int a = 0; a < ND; ++a) p0[a] = src[a] + t0 * vw[a];
 
-    while (true) {
-        float t1;
-        bool last = false;
-        if (!mg.pop(t_hi, t1)) {
-            if (close_at_one && t0 < 1.0f) {
-                t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
-                last = true;
+        while (true) {
+            float t1;
+            bool last = false;
+            if (!mg.template pop<EXH>(t_stop, t1)) {
+                if (close_at_one && t0 < 1.0f) {
+                    t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
+                    last = true;
+                } else {
+                    break;
+                }
+            }
+            float c[ND];
+            float len;
+            if (MODE == MODE_TORCH) {
+                float p1[ND];
+                float ss = 0.0f;
+#pragma unroll
+                for (int a = 0; a < ND; ++a) {
+                    p1[a] = src[a] + t1 * vw[a];
+                    const float dd = p1[a] - p0[a];
+                    const float sq = dd * dd;
+                    ss = (a == 0) ? sq : ss + sq;
+                }
+                len = seg_sqrt(ss);
+                mi.from_ends(p0, p1, c);
+#pragma unroll
+                for (int a = 0; a < ND; ++a) p0[a] = p1[a];
             } else {
-                break;
+                float m[ND];
+                const float tm = 0.5f * (t0 + t1);
+#pragma unroll
+                for (int a = 0; a < ND; ++a) m[a] = __builtin_fmaf(tm, vw[a], src[a]);
+                len = (t1 > t0) ? (t1 - t0) * rl : 0.0f;
+                mid_coords<ND, MODE, DIAG>(g, m, c);
             }
+            t0 = t1;
+            uint32_t idx;
+            const bool in = flat_voxel_bounded<ND>(win, c, &idx);
+            if (in && len > 0.0f) visit(idx, len);
+            if (last) break;
         }
-        float c[ND];
-        float len;
-        if (MODE == MODE_TORCH) {
-            float p1[ND];
-            float ss = 0.0f;
-#pragma unroll
-            for (int a = 0; a < ND; ++a) {
-                p1[a] = src[a] + t1 * vw[a];
-                const float dd = p1[a] - p0[a];
-                const float sq = dd * dd;
-                ss = (a == 0) ? sq : ss + sq;
-            }
-            len = seg_sqrt(ss);
-            mi.from_ends(p0, p1, c);
-#pragma unroll
-            for (int a = 0; a < ND; ++a) p0[a] = p1[a];
-        } else {
-            float m[ND];
-            const float tm = 0.5f * (t0 + t1);
-#pragma unroll
-            for (int a = 0; a < ND; ++a) m[a] = __builtin_fmaf(tm, vw[a], src[a]);
-            len = (t1 > t0) ? (t1 - t0) * rl : 0.0f;
-            mid_coords<ND, MODE, DIAG>(g, m, c);
-        }
-        t0 = t1;
-        uint32_t idx;
-        const bool in = flat_voxel_bounded<ND>(win, c, &idx);
-        if (in && len > 0.0f) visit(idx, len);
-        if (last) break;
-    }
+    };
+    if (e_max < 0.5f)
+        walk(std::integral_constant<bool, false>(), fminf(t_hi, t_cap));
+    else
+        walk(std::integral_constant<bool, true>(), t_hi);
 }
 
 // The whole grid: visit(flat_index, length) with the reference's flat index.

@@ -24,6 +24,15 @@ def random_rays(rng, shape, s, n, kind):
     if kind == "far":
         src = rng.uniform(-3, 3, (n, nd)) * ext
         dst = rng.uniform(-3, 3, (n, nd)) * ext
+    elif kind == "distant":
+        # endpoints ~4e4 voxels away on opposite sides, lines through
+        # the grid: midpoint error bounds above one half voxel
+        c = rng.uniform(-0.6, 0.6, (n, nd)) * ext
+        u = rng.normal(size=(n, nd))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        D = 40000.0 * float(np.max(s))  # ~4e4 voxels: E near 1.2
+        src = c + u * D
+        dst = c - u * D + rng.uniform(-0.5, 0.5, (n, nd)) * ext
     elif kind == "inside":
         src = rng.uniform(-0.6, 0.6, (n, nd)) * ext
         dst = rng.uniform(-0.6, 0.6, (n, nd)) * ext

@@ -50,7 +50,7 @@ def _assert_lists_equal(shape, minv, b, src, dst, mname, mode, dev, zero_rule=No
 
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("mname,mode", MODES)
-@pytest.mark.parametrize("kind", ["far", "inside", "lattice"])
+@pytest.mark.parametrize("kind", ["far", "inside", "lattice", "distant"])
 def test_segment_lists_bit_exact_random(dev, nd, mname, mode, kind):
     rng = np.random.default_rng(1000 + nd * 10 + mode)
     for trial in range(4):

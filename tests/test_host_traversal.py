@@ -54,9 +54,9 @@ def _check(host, shape, minv, b, src, dst, mode):
 @pytest.mark.parametrize("nd", [2, 3])
 def test_random_geometries(host, mode, nd):
     rng = np.random.default_rng(11 * nd + (mode == "otf"))
-    for trial in range(12):
+    for trial in range(16):
         shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial % 4 == 3))
-        kind = ["far", "inside", "lattice"][trial % 3]
+        kind = ["far", "inside", "lattice", "distant"][trial % 4]
         src, dst = random_rays(rng, shape, s, 1500, kind)
         _check(host, shape, minv, b, src, dst, mode)
 
@@ -118,9 +118,9 @@ def test_windows_partition_the_traversal(host, host_window, mode, nd):
     import itertools
 
     rng = np.random.default_rng(77 + nd)
-    for trial in range(6):
+    for trial in range(8):
         shape, M, minv, b, s = random_geometry(rng, nd, rotated=(trial == 5))
-        kind = ["far", "inside", "lattice"][trial % 3]
+        kind = ["far", "inside", "lattice", "distant"][trial % 4]
         src, dst = random_rays(rng, shape, s, 600, kind)
         zr = "exact" if (nd == 2 and mode == "torch") else "thresh"
         full = host(shape, minv, b, src, dst, mode, zr)
