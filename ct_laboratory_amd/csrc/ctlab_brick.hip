@@ -447,7 +447,7 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
                 s[a] = src[3ull * ray + a];
                 d[a] = dst[3ull * ray + a];
             }
-            trace_ray_window<3, MODE, DIAG>(
+            trace_ray_window<3, MODE, DIAG, false>(
                 g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
         }
         // the next chunk starts in the largest staged view whose prefix <= c1

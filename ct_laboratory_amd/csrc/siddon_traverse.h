@@ -448,10 +448,12 @@ struct MidIndex {
 // argument above, so each segment of the full traversal is visited by exactly
 // the window that contains its voxel.
 //
+// PAIR unrolls the segment loop by two (see walk below).
+//
 // e_bound >= 0 replaces the per-ray midpoint error bound E by a caller's bound
 // that holds for this ray on every axis (e.g. the maximum over a view): a
 // larger E only widens [t_lo, t_hi], which produces the same segments.
-template <int ND, int MODE, bool DIAG, class Visit>
+template <int ND, int MODE, bool DIAG, bool PAIR = true, class Visit>
 __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, const Window<ND>& win,
                                                           const float* src, const float* dst, Visit&& visit,
                                                           float e_bound = -1.0f) {
@@ -565,37 +567,31 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
             return;
         }
         const bool close_at_one = (MODE == MODE_OTF) && (t_stop == 1.0f);
-        float p0[ND];
-#pragma unroll
-        for (int a = 0; a < ND; ++a) p0[a] = src[a] + t0 * vw[a];
-
-        while (true) {
-            float t1;
-            bool last = false;
-            if (!mg.template pop<EXH>(t_stop, t1)) {
-                if (close_at_one && t0 < 1.0f) {
-                    t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
-                    last = true;
-                } else {
-                    break;
-                }
+        // Next segment end: the next crossing, or t = 1 closing an otf ray.
+        auto next = [&](float& t1, bool& last) -> bool {
+            if (mg.template pop<EXH>(t_stop, t1)) return true;
+            if (close_at_one && t0 < 1.0f) {
+                t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
+                last = true;
+                return true;
             }
+            return false;
+        };
+        // Segment [t0, t1]; pa holds P(t0) and pb receives P(t1) (torch rule).
+        auto segment = [&](const float* pa, float* pb, float t1) {
             float c[ND];
             float len;
             if (MODE == MODE_TORCH) {
-                float p1[ND];
                 float ss = 0.0f;
 #pragma unroll
                 for (int a = 0; a < ND; ++a) {
-                    p1[a] = src[a] + t1 * vw[a];
-                    const float dd = p1[a] - p0[a];
+                    pb[a] = src[a] + t1 * vw[a];
+                    const float dd = pb[a] - pa[a];
                     const float sq = dd * dd;
                     ss = (a == 0) ? sq : ss + sq;
                 }
                 len = seg_sqrt(ss);
-                mi.from_ends(p0, p1, c);
-#pragma unroll
-                for (int a = 0; a < ND; ++a) p0[a] = p1[a];
+                mi.from_ends(pa, pb, c);
             } else {
                 float m[ND];
                 const float tm = 0.5f * (t0 + t1);
@@ -608,6 +604,25 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
             uint32_t idx;
             const bool in = flat_voxel_bounded<ND>(win, c, &idx);
             if (in && len > 0.0f) visit(idx, len);
+        };
+        // PAIR: two segments per trip with the end-point registers swapping
+        // roles, so no point is copied (fewer instructions, more registers).
+        float p0[ND], p1[ND];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) p0[a] = src[a] + t0 * vw[a];
+        while (true) {
+            float t1;
+            bool last = false;
+            if (!next(t1, last)) break;
+            segment(p0, p1, t1);
+            if (last) break;
+            if (!PAIR) {
+#pragma unroll
+                for (int a = 0; a < ND; ++a) p0[a] = p1[a];
+                continue;
+            }
+            if (!next(t1, last)) break;
+            segment(p1, p0, t1);
             if (last) break;
         }
     };
