@@ -333,7 +333,8 @@ __device__ __forceinline__ uint32_t frame_estimate(const float* __restrict__ f, 
 // traced in bucket order, so the 64 lanes of a wave trace rays of similar
 // length instead of idling behind the longest one.  A chunk ends at the first
 // item beyond the staged views.
-constexpr int kChunk = 2 * kBrickThreads;
+constexpr int kPerThread = 4;  // items per thread and chunk
+constexpr int kChunk = kPerThread * kBrickThreads;  // queue entries are chunk offsets (< 2^12)
 constexpr int kBuckets = 8;
 constexpr int kStage = 64;
 
@@ -359,7 +360,7 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
                                             const int32_t* __restrict__ p, const float* __restrict__ frames,
                                             int n_view, int rows, int cols, uint32_t item_lo, uint32_t item_hi,
                                             uint32_t total, float e_brick, float s1,
-                                            float s2, uint32_t* queue, uint32_t* bcnt, int4* s_plan, float* s_frame,
+                                            float s2, uint16_t* queue, uint32_t* bcnt, int4* s_plan, float* s_frame,
                                             Add&& add) {
     const int tid = threadIdx.x;
     const bool e_ok = e_brick <= 8.0f;
@@ -398,19 +399,23 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
         __syncthreads();
         const uint32_t lim = (uint32_t)s_plan[kStage].w;  // first item beyond the staged views
         const uint32_t c1 = min(c0 + (uint32_t)kChunk, min(lim, item_hi));
-        uint32_t rk[2], key[2];
+        // item -> (staged view j, detector row r, column c, ray)
+        auto decode = [&](uint32_t item, int& j, int& r, int& c) -> uint32_t {
+            j = staged_view(s_plan, kStage, item);
+            const int4 e = s_plan[j];
+            const uint32_t local = item - (uint32_t)e.w, w = (uint32_t)e.z;
+            r = e.x + (int)(local / w);
+            c = e.y + (int)(local % w);
+            return (uint32_t)(((int64_t)(vs + j) * rows + r) * cols + c);
+        };
+        uint32_t key[kPerThread];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < kPerThread; ++k) {
             key[k] = ~0u;
-            rk[k] = 0u;
             const uint32_t item = c0 + (uint32_t)tid + (uint32_t)(k * kBrickThreads);
             if (item < c1) {
-                const int j = staged_view(s_plan, kStage, item);
-                const int4 e = s_plan[j];
-                const uint32_t local = item - (uint32_t)e.w, w = (uint32_t)e.z;
-                const int r = e.x + (int)(local / w), c = e.y + (int)(local % w);
-                const uint32_t ray = (uint32_t)(((int64_t)(vs + j) * rows + r) * cols + c);
-                rk[k] = ray;
+                int j, r, c;
+                const uint32_t ray = decode(item, j, r, c);
                 if (sn[ray] != 0.0f) {
                     const float* f = s_frame + j * kFrameFloats;
                     const uint32_t est = (f[F_VALID] != 0.0f)
@@ -435,12 +440,14 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (key[k] != ~0u) queue[bcnt[key[k] >> 12] + (key[k] & 0xfffu)] = rk[k];
+        for (int k = 0; k < kPerThread; ++k)
+            if (key[k] != ~0u)
+                queue[bcnt[key[k] >> 12] + (key[k] & 0xfffu)] = (uint16_t)(tid + k * kBrickThreads);
         __syncthreads();
         const uint32_t hits = bcnt[kBuckets];
         for (uint32_t q = tid; q < hits; q += kBrickThreads) {
-            const uint32_t ray = queue[q];
+            int j, r, c;
+            const uint32_t ray = decode(c0 + queue[q], j, r, c);
             const float val = sn[ray] * s1 * s2;
             float s[3], d[3];
             for (int a = 0; a < 3; ++a) {
@@ -540,7 +547,7 @@ __device__ __forceinline__ void brick_unit(
     const float* __restrict__ src,
     const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b,
     const float* __restrict__ view_max, int accumulate, int split, int bid, int part, int64_t bi,
-    unsigned long long* acc, uint32_t* queue, uint32_t* bcnt, float* s_red, int4* s_plan, float* s_frame) {
+    unsigned long long* acc, uint16_t* queue, uint32_t* bcnt, float* s_red, int4* s_plan, float* s_frame) {
     const int tid = threadIdx.x;
     const Window<3> win = brick_window(dims, bg, bid);
     for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
@@ -581,14 +588,14 @@ __device__ __forceinline__ void brick_unit(
     }
     if (tid % 64 == 0) {
         s_red[tid / 64] = vmax;
-        queue[tid / 64] = __float_as_uint(emax);
+        reinterpret_cast<float*>(queue)[tid / 64] = emax;  // the queue is free until the items start
     }
     __syncthreads();
     vmax = 0.0f;
     emax = 0.0f;
     for (int k = 0; k < kBrickThreads / 64; ++k) {
         vmax = fmaxf(vmax, s_red[k]);
-        emax = fmaxf(emax, __uint_as_float(queue[k]));
+        emax = fmaxf(emax, reinterpret_cast<const float*>(queue)[k]);
     }
     // Both are block-uniform: keep them in scalar registers.
     vmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmax)));
@@ -686,7 +693,7 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
     const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b,
     const float* __restrict__ view_max, int accumulate, int split) {
     __shared__ unsigned long long acc[kBrickLds];
-    __shared__ uint32_t queue[kChunk];
+    __shared__ __align__(16) uint16_t queue[kChunk];
     __shared__ uint32_t bcnt[kBuckets + 1];
     __shared__ float s_red[kBrickThreads / 64];
     __shared__ int4 s_plan[kStage + 1];
