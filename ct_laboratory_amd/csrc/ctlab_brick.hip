@@ -329,13 +329,13 @@ __device__ __forceinline__ uint32_t frame_estimate(const float* __restrict__ f, 
 // the block instead of a dependent chain per item); every thread then takes
 // two items, finds their views by a binary search in LDS, drops rays with a
 // zero value or no crossing of the brick, and files the rest into buckets by
-// their estimated crossing count (powers of two); then the chunk's rays are
+// their estimated crossing count (quarter octaves); then the chunk's rays are
 // traced in bucket order, so the 64 lanes of a wave trace rays of similar
 // length instead of idling behind the longest one.  A chunk ends at the first
 // item beyond the staged views.
 constexpr int kPerThread = 4;  // items per thread and chunk
 constexpr int kChunk = kPerThread * kBrickThreads;  // queue entries are chunk offsets (< 2^12)
-constexpr int kBuckets = 8;
+constexpr int kBuckets = 32;  // quarter-octave length classes (1, 1.25, 1.5, 1.75, 2, 2.5, ... crossings)
 constexpr int kStage = 64;
 
 // Largest j in [0, n) with prefix(j) <= item, for prefixes non-decreasing in j
@@ -422,7 +422,8 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
                                              ? frame_estimate(f, r, c, blo, bhi)
                                              : segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
                     if (est) {
-                        const uint32_t bk = min(32u - (uint32_t)__clz(est - 1u), (uint32_t)(kBuckets - 1));
+                        const uint32_t l2 = 31u - (uint32_t)__clz(est);  // est in [2^l2, 2^(l2+1))
+                        const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
                         key[k] = (bk << 12) | atomicAdd(&bcnt[bk], 1u);
                     }
                 }
