@@ -93,7 +93,7 @@ def resident_bytes(req: MAPRequest) -> int:
         geometry = 16 * n_ray + 4 * n_ray * (2 * req.n + 2)
     else:
         n_ray, vox = req.n_view * req.det * req.det, req.n ** 3
-        bricks = math.ceil(req.n / 32) * math.ceil(req.n / 16) ** 2
+        bricks = math.ceil(req.n / 32) ** 2 * math.ceil(req.n / 8)  # 32 x 32 x 8 bricks
         geometry = 24 * n_ray + 4 * (bricks * (4 * req.n_view + 1) + req.n_view) + 64 * req.n_view
     return int(geometry + 3 * 4 * n_ray + 5 * 4 * vox)
 
