@@ -96,7 +96,9 @@ def batched_bench(a, dev) -> int:
     """64 independent 2D scans (sizes 128^2-512^2, 90-720 views, half fan beam, half
     StaticCT ring; scan i seeded by default_rng(100 + i)): forward + back of all
     scans as one persistent launch each way (tuned) against 64 sequential
-    single-scan launches each way.  Prints one JSON line."""
+    single-scan launches each way.  Both back projections use the LDS tile
+    kernel; the persistent one-ray-per-lane back projection with global float
+    atomics is timed beside them.  Prints one JSON line."""
     import torch
 
     from ct_laboratory_amd import _C
@@ -138,16 +140,25 @@ def batched_bench(a, dev) -> int:
     t_bat = timed(batched, 5)
     out = bp.forward(images)
     same = all(torch.equal(out[k], sinos[k][0]) for k in range(len(scans)))
+    bp_atomic = BatchedProjector2D(scans, dev, rays_per_item=bp.rays_per_item, groups_per_cu=bp.groups_per_cu,
+                                   tiled_back=False)
+
+    def batched_atomic():
+        bp_atomic.back(bp_atomic.forward(images))
+
+    t_atomic = timed(batched_atomic, 3)
     res = {
         "metric": "64 uneven 2D scans, forward + back: aggregate rays/s, one persistent launch vs 64 launches",
         "n_scans": len(scans), "n_rays": n_rays,
         "classes": {c: sum(sc.geometry_class == c for sc in scans) for c in ("fan", "ring")},
         "sequential": {"ms": 1e3 * t_seq, "rays_per_s": n_rays / t_seq},
         "batched": {"ms": 1e3 * t_bat, "rays_per_s": n_rays / t_bat, "rays_per_item": best[0],
-                    "groups_per_cu": best[1]},
+                    "groups_per_cu": best[1], "rays_per_part": best[2]},
         "speedup": t_seq / t_bat,
+        "batched_atomic_back": {"ms": 1e3 * t_atomic, "rays_per_s": n_rays / t_atomic},
         "forward_bit_identical": same,
-        "tuner_ms": {f"{k[0]}x{k[1]}": v for k, v in sorted(timing.items())},
+        "tuner_ms": {"forward": {f"{k[0]}x{k[1]}": v for k, v in sorted(timing["forward"].items())},
+                     "back": {str(k): v for k, v in sorted(timing["back"].items())}},
     }
     print(json.dumps(res), flush=True)
     return 0

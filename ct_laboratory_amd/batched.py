@@ -2,12 +2,20 @@
 
 ``BatchedProjector2D`` holds the rays of every scan on the device and a
 table of ``ctlab_scan2d`` records.  A forward or back projection of the whole
-batch is one persistent launch (``ctlab_forward_project_2d_batch`` /
-``ctlab_back_project_2d_batch``): workgroups take work items of
-``rays_per_item`` consecutive rays from a per-batch queue.  ``autotune``
-chooses ``rays_per_item`` and the number of persistent workgroups per CU by
-timing the batch's own projections; results are cached per geometry-class
-signature of the batch.
+batch is one persistent launch:
+
+- forward (``ctlab_forward_project_2d_batch``): workgroups take work items of
+  ``rays_per_item`` consecutive rays from a per-batch queue, one ray per lane;
+- back (``ctlab_back_project_2d_batch_tiled``): work units are (64x64 image
+  tile, ``rays_per_part`` consecutive rays); a unit sums into LDS and writes
+  its tile once, so no segment costs a global atomic.  ``tiled_back=False``
+  selects the one-ray-per-lane atomic launch (``ctlab_back_project_2d_batch``)
+  instead.
+
+``autotune`` chooses ``rays_per_item`` and the number of persistent
+workgroups per CU for the forward, and ``rays_per_part`` for the back
+projection, by timing the batch's own projections; results are cached per
+geometry-class signature of the batch.
 
 Per scan the segments are those of the single-scan on-the-fly kernels
 (``_C.forward/back_project_2d_on_the_fly_cuda``), so the forward projections
@@ -33,6 +41,8 @@ SCAN_DTYPE = np.dtype([("image", "<u8"), ("sino", "<u8"), ("src", "<u8"), ("dst"
 assert SCAN_DTYPE.itemsize == 88
 
 N_CU = 256
+# Tiled back projection: workgroups of 1024 lanes with 41 KB of LDS, two per CU.
+TILED_GROUPS_PER_CU = 2
 
 
 class Scan2D:
@@ -63,7 +73,8 @@ class Scan2D:
 class BatchedProjector2D:
     _tuned: dict = {}
 
-    def __init__(self, scans, device=None, rays_per_item=1024, groups_per_cu=4, tuner_file=None):
+    def __init__(self, scans, device=None, rays_per_item=1024, groups_per_cu=4, tuner_file=None, rays_per_part=32768,
+                 tiled_back=True):
         self.scans = list(scans)
         self.device = torch.device("cuda") if device is None else torch.device(device)
         for sc in self.scans:
@@ -74,12 +85,14 @@ class BatchedProjector2D:
         self.images = [torch.empty(sc.n_row, sc.n_col, device=self.device) for sc in self.scans]
         self.tuner_file = tuner_file
         self.rays_per_item, self.groups_per_cu = int(rays_per_item), int(groups_per_cu)
+        self.rays_per_part, self.tiled_back = int(rays_per_part), bool(tiled_back)
         key = self.class_signature()
         cached = self._load_tuned().get(key)
         if cached:
-            self.rays_per_item, self.groups_per_cu = cached
-        self._table = None
-        self._table_key = None
+            self.rays_per_item, self.groups_per_cu = cached[:2]
+            if len(cached) > 2:
+                self.rays_per_part = cached[2]
+        self._tables = {}
 
     # -- tuner cache ----------------------------------------------------------
     def class_signature(self) -> str:
@@ -95,7 +108,10 @@ class BatchedProjector2D:
         return BatchedProjector2D._tuned
 
     # -- descriptor table -------------------------------------------------------
-    def _build(self, images, sinos, rays_per_item):
+    def _build(self, images, sinos, rays_per_item, tiled=False):
+        """Descriptor table and total work count: items of rays_per_item rays,
+        or (tiled) units of 64x64 tiles x rays_per_item rays."""
+        lib = N.load() if tiled else None
         rec = np.zeros(len(self.scans), SCAN_DTYPE)
         begin = 0
         for k, sc in enumerate(self.scans):
@@ -108,26 +124,38 @@ class BatchedProjector2D:
             rec[k]["n_row"], rec[k]["n_col"], rec[k]["n_ray"] = sc.n_row, sc.n_col, sc.n_ray
             rec[k]["mode"], rec[k]["zero_rule"] = sc.mode, sc.zero_rule
             rec[k]["item_begin"] = begin
-            begin += -(-sc.n_ray // rays_per_item)
+            if tiled:
+                begin += int(lib.ctlab_back_project_2d_tile_units(sc.n_row, sc.n_col, sc.n_ray, rays_per_item))
+            else:
+                begin += -(-sc.n_ray // rays_per_item)
         table = torch.from_numpy(rec.view(np.uint8).copy()).to(self.device)
         return table, begin
 
-    def _launch(self, back, images, sinos, rays_per_item, groups_per_cu, accumulate=False):
+    def _launch(self, back, images, sinos, rays_per_item, groups_per_cu, accumulate=False, tiled=False):
+        """One persistent launch; for a tiled back projection rays_per_item is
+        the rays per part."""
         for t in list(images) + list(sinos):
             if t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device:
                 raise RuntimeError("batched projector: images and sinograms must be contiguous float32 on the device")
-        key = (tuple(t.data_ptr() for t in images), tuple(t.data_ptr() for t in sinos), rays_per_item)
-        if key != self._table_key:
-            self._table, self._items = self._build(images, sinos, rays_per_item)
-            self._table_key = key
-            self._table_refs = (list(images), list(sinos))  # keep the pointed-to tensors alive
+        tiled = bool(back and tiled)
+        key = (tuple(t.data_ptr() for t in images), tuple(t.data_ptr() for t in sinos), rays_per_item, tiled)
+        if key not in self._tables:
+            table, items = self._build(images, sinos, rays_per_item, tiled)
+            # the pointed-to tensors stay referenced as long as the table
+            self._tables[key] = (table, items, list(images), list(sinos))
+            while len(self._tables) > 8:
+                self._tables.pop(next(iter(self._tables)))
+        table, items = self._tables[key][:2]
         stream = N.stream_of(self.counter)
         with torch.cuda.device(self.device):
-            if back:
-                N.call("ctlab_back_project_2d_batch", N.ptr(self._table), len(self.scans), self._items, rays_per_item,
+            if tiled:
+                N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(self.scans), items, rays_per_item,
+                       TILED_GROUPS_PER_CU * N_CU, N.ptr(self.counter), int(bool(accumulate)), stream)
+            elif back:
+                N.call("ctlab_back_project_2d_batch", N.ptr(table), len(self.scans), items, rays_per_item,
                        groups_per_cu * N_CU, N.ptr(self.counter), int(bool(accumulate)), stream)
             else:
-                N.call("ctlab_forward_project_2d_batch", N.ptr(self._table), len(self.scans), self._items,
+                N.call("ctlab_forward_project_2d_batch", N.ptr(table), len(self.scans), items,
                        rays_per_item, groups_per_cu * N_CU, N.ptr(self.counter), stream)
 
     # -- projections ------------------------------------------------------------
@@ -144,34 +172,49 @@ class BatchedProjector2D:
         for sc, s in zip(self.scans, sinos):
             if tuple(s.shape) != (sc.n_ray,):
                 raise RuntimeError("batched projector: sinogram shape does not match its scan")
-        self._launch(True, self.images, sinos, self.rays_per_item, self.groups_per_cu)
+        if self.tiled_back:
+            self._launch(True, self.images, sinos, self.rays_per_part, 0, tiled=True)
+        else:
+            self._launch(True, self.images, sinos, self.rays_per_item, self.groups_per_cu)
         return self.images
 
+    def _time(self, fn, reps):
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        ev[1].synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
     def autotune(self, images, rays_per_item=(256, 512, 1024, 2048, 4096), groups_per_cu=(1, 2, 4, 8, 16),
-                 reps=3):
-        """Times forward + back of this batch for every (rays_per_item,
-        groups_per_cu) and keeps the fastest; returns (best, {config: ms})."""
-        timing = {}
+                 rays_per_part=(8192, 16384, 32768, 65536, 131072), reps=3):
+        """Times this batch's forward for every (rays_per_item, groups_per_cu),
+        and its back projection for every rays_per_part (tiled) or every
+        (rays_per_item, groups_per_cu) (atomic), and keeps the fastest.
+        Returns ((rays_per_item, groups_per_cu, rays_per_part), {"forward":
+        {config: ms}, "back": {config: ms}})."""
         sinos = [torch.empty(sc.n_ray, device=self.device) for sc in self.scans]
-        for rpi, gpc in itertools.product(rays_per_item, groups_per_cu):
-            self._launch(False, images, sinos, rpi, gpc)
-            self._launch(True, self.images, sinos, rpi, gpc)
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record()
-            for _ in range(reps):
-                self._launch(False, images, sinos, rpi, gpc)
-                self._launch(True, self.images, sinos, rpi, gpc)
-            ev[1].record()
-            ev[1].synchronize()
-            timing[(rpi, gpc)] = ev[0].elapsed_time(ev[1]) / reps
-        best = min(timing, key=timing.get)
-        self.rays_per_item, self.groups_per_cu = best
+        fwd = {(rpi, gpc): self._time(lambda: self._launch(False, images, sinos, rpi, gpc), reps)
+               for rpi, gpc in itertools.product(rays_per_item, groups_per_cu)}
+        if self.tiled_back:
+            back = {rpp: self._time(lambda: self._launch(True, self.images, sinos, rpp, 0, tiled=True), reps)
+                    for rpp in rays_per_part}
+            best_fwd, best_rpp = min(fwd, key=fwd.get), min(back, key=back.get)
+        else:
+            back = {(rpi, gpc): self._time(lambda: self._launch(True, self.images, sinos, rpi, gpc), reps)
+                    for rpi, gpc in itertools.product(rays_per_item, groups_per_cu)}
+            # one (rays_per_item, groups_per_cu) serves both launches
+            best_fwd = min(fwd, key=lambda k: fwd[k] + back[k])
+            best_rpp = self.rays_per_part
+        best = (best_fwd[0], best_fwd[1], best_rpp)
+        self.rays_per_item, self.groups_per_cu, self.rays_per_part = best
         BatchedProjector2D._tuned[self.class_signature()] = list(best)
         if self.tuner_file:
             with open(self.tuner_file, "w") as f:
                 json.dump(BatchedProjector2D._tuned, f)
-        self._table_key = None
-        return best, timing
+        return best, {"forward": fwd, "back": back}
 
 
 def c4_workload(n_scan=64, device=None):

@@ -96,6 +96,13 @@ __global__ __launch_bounds__(kBlock) void k_batch2d_clear(const ctlab_scan2d* __
     }
 }
 
+int batch2d_clear(const char* fn, const ctlab_scan2d* scans, int64_t n_scan, bool clear_images,
+                  unsigned long long* counter, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch2d_clear, dim3(64, (unsigned)(n_scan < 1024 ? n_scan : 1024)), dim3(kBlock), 0, stream,
+                       scans, (int)n_scan, clear_images ? 1 : 0, counter);
+    return check_launch(fn);
+}
+
 static int batch2d(const char* fn, const ctlab_scan2d* scans, int64_t n_scan, int64_t total_items,
                    int64_t rays_per_item, int64_t n_groups, void* counter, bool back, int accumulate,
                    hipStream_t stream) {
@@ -105,9 +112,7 @@ static int batch2d(const char* fn, const ctlab_scan2d* scans, int64_t n_scan, in
     if (n_scan == 0) return CTLAB_OK;
     CTLAB_REQUIRE(scans && counter, "%s: null pointer", fn);
     auto* ctr = reinterpret_cast<unsigned long long*>(counter);
-    hipLaunchKernelGGL(k_batch2d_clear, dim3(64, (unsigned)(n_scan < 1024 ? n_scan : 1024)), dim3(kBlock), 0, stream,
-                       scans, (int)n_scan, back && !accumulate ? 1 : 0, ctr);
-    int st = check_launch(fn);
+    int st = batch2d_clear(fn, scans, n_scan, back && !accumulate, ctr, stream);
     if (st) return st;
     if (total_items == 0) return CTLAB_OK;
     if (back)

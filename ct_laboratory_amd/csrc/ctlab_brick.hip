@@ -28,6 +28,7 @@
 // flat-panel model within a quarter pixel (source moving, detector not
 // affine) is marked invalid and uses its whole detector for every brick.
 #include "ctlab_common.h"
+#include "fixed_point.h"
 #include "siddon_traverse.h"
 
 namespace ctlab {
@@ -465,27 +466,6 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
     }
 }
 
-// Round-half-even of c (|c| < 2^50) to a 64-bit two's-complement integer: c
-// is exact in double, and adding 1.5 * 2^52 lands in [2^52, 2^53), where the
-// double spacing is 1, so that addition rounds c half-to-even and the integer
-// sits in the low bits of the sum's representation.
-__device__ __forceinline__ unsigned long long to_fixed(float c) {
-    const double y = (double)c + 6755399441055744.0;  // 1.5 * 2^52
-    return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
-}
-
-// *p += v with the shader's IEEE single addition (compare-and-swap loop; each
-// failed swap means another writer's add landed, so the loop ends).
-__device__ __forceinline__ void add_f32_ieee(float* p, float v) {
-    unsigned int* q = reinterpret_cast<unsigned int*>(p);
-    unsigned int old = *q;
-    while (true) {
-        const unsigned int seen = atomicCAS(q, old, __float_as_uint(__uint_as_float(old) + v));
-        if (seen == old) break;
-        old = seen;
-    }
-}
-
 // Longest segment any ray can have inside one voxel: the largest body
 // diagonal of the voxel parallelepiped, |M d| for d in {(1,1,1), (1,1,-1),
 // (1,-1,1), (-1,1,1)}, with M the inverse of M_inv.  Evaluated in double.
@@ -614,7 +594,10 @@ __device__ __forceinline__ void brick_unit(
     // [-250, 250] is exact.  The absolute error of a voxel is at most half a
     // quantum 2^-F per contribution.  Non-finite data, or products that could
     // overflow a float, take float atomics into global memory instead (INF/NaN
-    // propagate as with the reference's atomics).
+    // propagate as with the reference's atomics).  (fixed_scale in
+    // fixed_point.h is the same computation; it is written out here because
+    // the brick kernel sits at its register budget and the call's different
+    // instruction order costs it spills.)
     const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
     const bool fixed = g.finite && cmax <= 1.0e38;
     const double bound = cmax * (double)(total > 0 ? total : 1);

@@ -48,6 +48,17 @@ inline int check_index_limits(const char* fn, const int64_t* n, int nd) {
 // that the call is safe inside a captured graph).
 int fill_f32(float* p, int64_t n, float value, hipStream_t stream);
 
+// Zeroes the images of a ctlab_scan2d table (when clear_images) and the work
+// counter of a persistent multi-scan launch.
+int batch2d_clear(const char* fn, const ctlab_scan2d* scans, int64_t n_scan, bool clear_images,
+                  unsigned long long* counter, hipStream_t stream);
+
+// 2D back projection of one scan (batch elements along the sinogram's first
+// axis) by the LDS tile kernel; the caller has checked the arguments.
+int back_project_2d_tiles(const char* fn, const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                          const float* dst, const float* minv, const float* b, const int64_t* n, int mode,
+                          int zero_rule, int accumulate, float* out, hipStream_t stream);
+
 }  // namespace ctlab
 
 #define CTLAB_REQUIRE(cond, ...)                                   \

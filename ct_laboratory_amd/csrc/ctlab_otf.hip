@@ -254,6 +254,13 @@ static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_
     st = make_sched(fn, n_ray, rows, cols, &sched);
     if (st) return st;
     CTLAB_REQUIRE(out, "%s: out is null", fn);
+    if constexpr (ND == 2) {  // atomic-free: LDS tiles (ctlab_tile2d.hip)
+        if (n_ray > 0) {
+            CTLAB_REQUIRE(sino, "%s: sino is null", fn);
+            return back_project_2d_tiles(fn, sino, batch, n_ray, src, dst, minv, b, n, mode, zero_rule, accumulate,
+                                         out, stream);
+        }
+    }
     int64_t V = 1;
     for (int a = 0; a < ND; ++a) V *= n[a];
     if (!accumulate) {

@@ -1,0 +1,326 @@
+// ctlab_tile2d.hip -- atomic-free 2D back projection over image tiles, for any
+// ray geometry (fan beam, static rings, parallel beams).
+//
+// A work unit is (tile, part): a 64x64 tile of one image and a contiguous
+// range of that scan's rays.  The unit's workgroup sums the range's
+// contributions to the tile in LDS as 64-bit fixed point (exact and
+// independent of the order of the rays, as in the 3D brick back projector)
+// and then writes the tile once.  Per chunk of 4096 rays, a conservative slab
+// test against the tile widened by the ray's rounding bound plus one pixel
+// keeps the rays that can reach it, compacted in ray order into an LDS queue
+// so that a wave's lanes trace neighbouring rays of similar length; each kept
+// ray is traced with the exact traversal clipped to the tile
+// (trace_ray_window), so the tile receives exactly the segments whose pixel it
+// contains.  Global float atomics, which execute past L2 and bound the
+// one-ray-per-lane back projection, are left only for combining parts: a tile
+// with one part is written with plain stores (deterministic); with several,
+// each part adds each pixel once with a float atomic.
+#include "ctlab_common.h"
+#include "fixed_point.h"
+#include "siddon_traverse.h"
+
+namespace ctlab {
+
+constexpr int kTile = 64;                      // tile edge, pixels
+constexpr int kTileStride = kTile + 1;         // LDS row stride (64-bit cells), odd
+constexpr int kTileCells = kTile * kTileStride;
+constexpr int kTileThreads = 1024;             // 16 waves; two workgroups per CU
+constexpr int kTilePerThread = 4;
+constexpr int kTileChunk = kTilePerThread * kTileThreads;  // rays filtered per chunk (< 2^16)
+
+static_assert(sizeof(ctlab_scan2d) == 88, "ctlab_scan2d layout");
+
+struct TileLds {
+    unsigned long long acc[kTileCells];
+    uint16_t queue[kTileChunk];
+    float red[kTileThreads / 64];
+    uint32_t count;
+};
+
+// Longest segment any ray can have inside one pixel: the longer diagonal of
+// the pixel parallelogram, |M d| for d in {(1,1), (1,-1)}, M = inverse of
+// M_inv.  Evaluated in double.
+__device__ __forceinline__ double pixel_diagonal(const Grid<2>& g) {
+    const double a = g.minv[0], b = g.minv[1], c = g.minv[2], d = g.minv[3];
+    const double det = a * d - b * c;
+    const double p0 = (d - b) / det, p1 = (a - c) / det;  // M (1, 1)
+    const double q0 = (d + b) / det, q1 = (-c - a) / det; // M (1, -1)
+    return fmax(sqrt(p0 * p0 + p1 * p1), sqrt(q0 * q0 + q1 * q1));
+}
+
+// Conservative test: can ray s -> d have a segment whose rounded pixel lies in
+// the window?  The traversal visits only segments whose rounded midpoint is in
+// the window; a midpoint coordinate is within E of the exact one (E as in
+// trace_ray_window, which drops the ray when E > 8), so such a ray passes
+// through the window widened by E + 1/2.  The slab test here widens it by
+// E + 1: the extra half pixel covers this test's own float rounding (relative
+// 2^-22 of coordinates below 2^20 pixels).  It drops rays only for E > 8.5, so
+// a rounding difference from the traversal's own E near 8 keeps the ray.
+template <int MODE, bool DIAG>
+__device__ __forceinline__ bool tile_may_hit(const Grid<2>& g, const Window<2>& win, const float* s,
+                                             const float* d) {
+    float sv[2], dv[2];
+    to_voxel_t<2, MODE, DIAG>(g, s, sv);
+    to_voxel_t<2, MODE, DIAG>(g, d, dv);
+    float t0 = 0.0f, t1 = 1.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        float mag = fabsf(sv[a]) + fabsf(dv[a]) + (float)g.n[a] + 1.0f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) mag += fabsf(g.minv[a * 2 + j]) * (fabsf(s[j]) + fabsf(d[j]) + fabsf(g.b[j]));
+        const float m = mag * 7.62939453125e-06f + 1.0f;  // E + 1 pixel, E = mag * 2^-17
+        if (!(m <= 9.5f)) return false;  // E well above the traversal's limit of 8, or NaN: no segments
+        const float lo = (float)win.lo[a] - 0.5f - m, hi = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
+        const float dir = dv[a] - sv[a];
+        if (fabsf(dir) < 1.0e-20f) {
+            if (!(sv[a] >= lo && sv[a] <= hi)) return false;
+            continue;
+        }
+        const float ta = (lo - sv[a]) / dir, tb = (hi - sv[a]) / dir;
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    }
+    return t0 <= t1;
+}
+
+// Block-wide maximum (all kTileThreads threads call it).
+__device__ __forceinline__ float tile_block_max(float v, float* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int k = 1; k < kTileThreads / 64; ++k) r = fmaxf(r, red[k]);
+    __syncthreads();  // red is reused
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r)));
+}
+
+// One work unit: rays [r0, r1) of a scan into tile `tile` of its image.
+// direct: this part is the tile's only one (plain stores; else float atomics
+// into an image that is zeroed or accumulated into).
+template <int MODE, bool DIAG>
+__device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const float* __restrict__ src,
+                          const float* __restrict__ dst, float* __restrict__ img, int tile, int tiles_c, int64_t r0,
+                          int64_t r1, bool direct, int accumulate, TileLds& L) {
+    const int tid = threadIdx.x;
+    Window<2> win;
+    win.lo[0] = (tile / tiles_c) * kTile;
+    win.lo[1] = (tile % tiles_c) * kTile;
+    win.ext[0] = min(kTile, g.n[0] - win.lo[0]);
+    win.ext[1] = min(kTile, g.n[1] - win.lo[1]);
+    win.stride[0] = kTileStride;
+    win.stride[1] = 1;
+    const int ec = win.ext[1], npx = win.ext[0] * ec;
+    auto gidx = [&](int i) -> int64_t {
+        return (int64_t)(win.lo[0] + i / ec) * g.n[1] + (win.lo[1] + i % ec);
+    };
+
+    for (int i = tid; i < kTileCells; i += kTileThreads) L.acc[i] = 0ull;
+
+    // 1. bound on |value| over the part's rays (INF when one is INF or NaN)
+    float vmax = 0.0f;
+    bool bad = false;
+    for (int64_t r = r0 + tid; r < r1; r += kTileThreads) {
+        const float a = fabsf(sn[r]);
+        bad |= !(a <= 3.0e38f);
+        vmax = fmaxf(vmax, a);
+    }
+    vmax = tile_block_max(bad ? __builtin_huge_valf() : vmax, L.red);
+
+    // 2. power-of-two scale: a ray's segments in one pixel add up to at most
+    // the pixel diagonal
+    const FixedScale fs = fixed_scale(2.0 * (double)vmax * pixel_diagonal(g), (double)(r1 - r0), g.finite);
+
+    if (!fs.fixed && direct && !accumulate) {  // float atomics below need a zeroed tile
+        for (int i = tid; i < npx; i += kTileThreads) img[gidx(i)] = 0.0f;
+    }
+    __syncthreads();
+
+    // 3. filter, queue and trace, chunk by chunk
+    if (vmax != 0.0f) {
+        const float s1 = fs.s1, s2 = fs.s2;
+        const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        for (int64_t c0 = r0; c0 < r1; c0 += kTileChunk) {
+            if (tid == 0) L.count = 0u;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kTilePerThread; ++k) {
+                const int off = tid + k * kTileThreads;
+                const int64_t r = c0 + off;
+                bool hit = false;
+                if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
+                    const float s[2] = {src[2 * r], src[2 * r + 1]};
+                    const float d[2] = {dst[2 * r], dst[2 * r + 1]};
+                    hit = tile_may_hit<MODE, DIAG>(g, win, s, d);
+                }
+                const uint64_t mask = __ballot(hit);
+                const uint32_t before =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                uint32_t base = 0u;
+                if (lane == 0 && mask != 0ull) base = atomicAdd(&L.count, (uint32_t)__popcll(mask));
+                base = __shfl(base, 0);
+                if (hit) L.queue[base + before] = (uint16_t)off;
+            }
+            __syncthreads();
+            const uint32_t hits = L.count;
+            for (uint32_t q = tid; q < hits; q += kTileThreads) {
+                const int64_t r = c0 + L.queue[q];
+                const float s[2] = {src[2 * r], src[2 * r + 1]};
+                const float d[2] = {dst[2 * r], dst[2 * r + 1]};
+                if (fs.fixed) {
+                    const float val = sn[r] * s1 * s2;
+                    trace_ray_window<2, MODE, DIAG, false>(
+                        g, win, s, d, [&](uint32_t li, float len) { atomicAdd(&L.acc[li], to_fixed(val * len)); });
+                } else {
+                    const float val = sn[r];
+                    trace_ray_window<2, MODE, DIAG, false>(g, win, s, d, [&](uint32_t li, float len) {
+                        const int i = (int)(li / kTileStride) * ec + (int)(li % kTileStride);
+                        unsafeAtomicAdd(img + gidx(i), val * len);
+                    });
+                }
+            }
+            __syncthreads();  // queue and count are reused
+        }
+    }
+
+    // 4. write the tile (rows of the image, coalesced)
+    if (!fs.fixed) return;
+    const double inv = ldexp(1.0, -fs.F);
+    for (int i = tid; i < npx; i += kTileThreads) {
+        const float v = (float)((double)(long long)L.acc[(i / ec) * kTileStride + i % ec] * inv);
+        float* o = img + gidx(i);
+        if (direct)
+            *o = accumulate ? *o + v : v;
+        else if (v != 0.0f)
+            unsafeAtomicAdd(o, v);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void tile_unit_any(const Grid<2>& g, const float* sn, const float* src, const float* dst,
+                                              float* img, int tile, int tiles_c, int64_t r0, int64_t r1, bool direct,
+                                              int accumulate, TileLds& L) {
+    if (g.diag)
+        tile_unit<MODE, true>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+    else
+        tile_unit<MODE, false>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+}
+
+// One scan, batch elements along y: unit x = part * n_tiles + tile (the tiles
+// of one ray range run together and share its rays in L2).
+template <int MODE>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_tile2d(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, const float* __restrict__ src,
+    const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b, int n_row, int n_col,
+    int zero_rule, int tiles_c, int n_tiles, int64_t rays_per_part, int n_parts, int accumulate) {
+    __shared__ TileLds L;
+    const int n[2] = {n_row, n_col};
+    const Grid<2> g = load_grid<2>(n, minv, b, zero_rule);
+    const int part = blockIdx.x / n_tiles, tile = blockIdx.x % n_tiles;
+    const int64_t r0 = (int64_t)part * rays_per_part, r1 = min(r0 + rays_per_part, n_ray);
+    const int64_t bi = blockIdx.y;
+    tile_unit_any<MODE>(g, sino + bi * n_ray, src, dst, out + bi * (int64_t)n_row * n_col, tile, tiles_c, r0, r1,
+                        n_parts == 1, accumulate, L);
+}
+
+// Many scans, persistent: workgroups take units from a counter.  Scan k owns
+// units [item_begin_k, item_begin_k + tiles_k * parts_k), unit = part * tiles_k
+// + tile, parts_k = ceil(n_ray_k / rays_per_part).
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2d_tiles(
+    const ctlab_scan2d* __restrict__ scans, int n_scan, int64_t total_units, int64_t rays_per_part, int accumulate,
+    unsigned long long* __restrict__ counter) {
+    __shared__ TileLds L;
+    __shared__ int64_t s_unit;
+    while (true) {
+        if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t unit = s_unit;
+        __syncthreads();  // every lane has read s_unit before it is overwritten
+        if (unit >= total_units) break;  // uniform: every workgroup reaches this exit
+        int lo = 0, hi = n_scan;          // largest k with item_begin <= unit
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (scans[mid].item_begin <= unit) lo = mid; else hi = mid;
+        }
+        const ctlab_scan2d sc = scans[lo];
+        const int n[2] = {sc.n_row, sc.n_col};
+        const Grid<2> g = load_grid<2>(n, sc.minv, sc.b, sc.zero_rule);
+        const int tiles_c = (sc.n_col + kTile - 1) / kTile;
+        const int n_tiles = ((sc.n_row + kTile - 1) / kTile) * tiles_c;
+        const int64_t local = unit - sc.item_begin;
+        const int64_t part = local / n_tiles;
+        const int tile = (int)(local % n_tiles);
+        const int64_t r0 = part * rays_per_part, r1 = min(r0 + rays_per_part, sc.n_ray);
+        if (r0 >= r1) continue;  // past the scan's units (a table with spare units): nothing to do
+        const bool direct = sc.n_ray <= rays_per_part;
+        const float* sn = reinterpret_cast<const float*>(sc.sino);
+        const float* s = reinterpret_cast<const float*>(sc.src);
+        const float* d = reinterpret_cast<const float*>(sc.dst);
+        float* img = reinterpret_cast<float*>(sc.image);
+        if (sc.mode == MODE_OTF)
+            tile_unit_any<MODE_OTF>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+        else
+            tile_unit_any<MODE_TORCH>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+        __syncthreads();  // the tile's LDS is cleared by the next unit
+    }
+}
+
+// Single-scan 2D back projection (ctlab_back_project_2d_otf): sizes are
+// checked by the caller.  Rays split into parts so that a launch has about
+// 1024 units of at least one chunk each.
+int back_project_2d_tiles(const char* fn, const float* sino, int64_t batch, int64_t n_ray, const float* src,
+                          const float* dst, const float* minv, const float* b, const int64_t* n, int mode,
+                          int zero_rule, int accumulate, float* out, hipStream_t stream) {
+    const int tiles_c = (int)((n[1] + kTile - 1) / kTile);
+    const int64_t n_tiles = ((n[0] + kTile - 1) / kTile) * tiles_c;
+    CTLAB_REQUIRE(n_tiles <= (1LL << 24), "%s: image too large for the tile grid", fn);
+    CTLAB_REQUIRE(n_ray >= 1, "%s: no rays", fn);
+    int64_t parts = (1024 + n_tiles * batch - 1) / (n_tiles * batch);  // about 1024 units
+    const int64_t max_parts = (n_ray + kTileChunk - 1) / kTileChunk;     // at least a chunk each
+    if (parts > max_parts) parts = max_parts;
+    if (parts < 1) parts = 1;
+    const int64_t rpp = (n_ray + parts - 1) / parts;
+    const int64_t n_parts = (n_ray + rpp - 1) / rpp;
+    CTLAB_REQUIRE(n_parts * n_tiles <= kMaxGridX, "%s: too many work units", fn);
+    if (n_parts > 1 && !accumulate) {
+        int st = fill_f32(out, n[0] * n[1] * batch, 0.0f, stream);
+        if (st) return st;
+    }
+    dim3 grid((unsigned)(n_parts * n_tiles), (unsigned)batch);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL(k_back_tile2d<MODE_TORCH>, grid, dim3(kTileThreads), 0, stream, sino, out, n_ray, src, dst,
+                           minv, b, (int)n[0], (int)n[1], zero_rule, tiles_c, (int)n_tiles, rpp, (int)n_parts,
+                           accumulate);
+    else
+        hipLaunchKernelGGL(k_back_tile2d<MODE_OTF>, grid, dim3(kTileThreads), 0, stream, sino, out, n_ray, src, dst,
+                           minv, b, (int)n[0], (int)n[1], zero_rule, tiles_c, (int)n_tiles, rpp, (int)n_parts,
+                           accumulate);
+    return check_launch(fn);
+}
+
+}  // namespace ctlab
+
+using namespace ctlab;
+
+extern "C" int64_t ctlab_back_project_2d_tile_units(int64_t n_row, int64_t n_col, int64_t n_ray,
+                                                     int64_t rays_per_part) {
+    if (n_row < 1 || n_col < 1 || n_ray < 0 || rays_per_part < 1) return -1;
+    const int64_t tiles = ((n_row + kTile - 1) / kTile) * ((n_col + kTile - 1) / kTile);
+    return tiles * ((n_ray + rays_per_part - 1) / rays_per_part);
+}
+
+extern "C" int ctlab_back_project_2d_batch_tiled(const ctlab_scan2d* scans, int64_t n_scan, int64_t total_units,
+                                                 int64_t rays_per_part, int64_t n_groups, void* counter,
+                                                 int accumulate, ctlab_stream_t stream) {
+    const char* fn = "back_project_2d_batch_tiled";
+    CTLAB_REQUIRE(n_scan >= 0 && n_scan <= 0x7fffffffLL, "%s: n_scan out of range", fn);
+    CTLAB_REQUIRE(total_units >= 0 && rays_per_part >= 1 && rays_per_part <= (1LL << 40), "%s: bad work units", fn);
+    CTLAB_REQUIRE(n_groups >= 1 && n_groups <= 65536, "%s: n_groups %lld out of range", fn, (long long)n_groups);
+    if (n_scan == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(scans && counter, "%s: null pointer", fn);
+    auto* ctr = reinterpret_cast<unsigned long long*>(counter);
+    int st = batch2d_clear(fn, scans, n_scan, !accumulate, ctr, as_stream(stream));
+    if (st || total_units == 0) return st;
+    hipLaunchKernelGGL(k_batch2d_tiles, dim3((unsigned)n_groups), dim3(kTileThreads), 0, as_stream(stream), scans,
+                       (int)n_scan, total_units, rays_per_part, accumulate, ctr);
+    return check_launch(fn);
+}

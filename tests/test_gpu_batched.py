@@ -1,11 +1,14 @@
 """GPU tests of the persistent multi-scan 2D launch: per-scan forward projections
-bit-identical to the single-scan kernel, back projections within 1e-5 (float
-atomics), queue exhaustion with uneven scans, and the tuner."""
+bit-identical to the single-scan kernel, back projections (LDS tiles and the
+atomic launch) within 1e-5 of the single-scan kernel and of the oracle, queue
+exhaustion with uneven scans, and the tuner."""
 import numpy as np
 import pytest
 import torch
 
+import oracle as O
 from ct_laboratory_amd import _C
+from ct_laboratory_amd import _native as N
 from ct_laboratory_amd.batched import BatchedProjector2D, Scan2D
 from ct_laboratory_amd.phantoms import ellipse_phantom_2d
 from ct_laboratory_amd.tomography import (build_fanbeam_2d_geometry, build_static_2d_geometry,
@@ -43,10 +46,12 @@ def _single(sc, img, dev):
     return _C.forward_project_2d_on_the_fly_cuda(img, sc.src, sc.dst, minv, b, mode=sc.mode, zero_rule=sc.zero_rule)[0]
 
 
-@pytest.mark.parametrize("rpi,gpc", [(256, 1), (512, 4), (4096, 16)])
-def test_batch_matches_single_scan_kernels(dev, rpi, gpc):
+@pytest.mark.parametrize("rpi,gpc,rpp,tiled", [(256, 1, 64, True), (512, 4, 4096, True), (4096, 16, 1 << 20, True),
+                                              (512, 4, 0, False)])
+def test_batch_matches_single_scan_kernels(dev, rpi, gpc, rpp, tiled):
     scans, images = _scans(dev)
-    bp = BatchedProjector2D(scans, dev, rays_per_item=rpi, groups_per_cu=gpc)
+    bp = BatchedProjector2D(scans, dev, rays_per_item=rpi, groups_per_cu=gpc, rays_per_part=max(rpp, 1),
+                            tiled_back=tiled)
     sinos = bp.forward(images)
     ys = []
     for sc, img, s in zip(scans, images, sinos):
@@ -58,6 +63,10 @@ def test_batch_matches_single_scan_kernels(dev, rpi, gpc):
         ref = _C.back_project_2d_on_the_fly_cuda(y, sc.src, sc.dst, sc.minv.to(dev), sc.b.to(dev), sc.n_row,
                                                  sc.n_col, mode=sc.mode, zero_rule=sc.zero_rule)[0]
         assert rel_err(bk.cpu().numpy(), ref.cpu().numpy()) < 1e-5
+        ref_o = O.back(y.cpu().numpy()[None], (sc.n_row, sc.n_col), sc.minv.numpy(), sc.b.numpy(),
+                       sc.src.cpu().numpy(), sc.dst.cpu().numpy(), mode="otf" if sc.mode == N.MODE_OTF else "torch",
+                       zero_rule="exact" if sc.zero_rule == N.ZERO_EXACT else "thresh")[0]
+        assert rel_err(bk.cpu().numpy(), ref_o) < 1e-5
     # repeated launches reuse the resident table and clear the queue counter
     again = bp.forward(images)
     assert all(torch.equal(a, _single(sc, im, dev)) for a, sc, im in zip(again, scans, images))
@@ -67,9 +76,11 @@ def test_autotune_and_cache(dev, tmp_path):
     scans, images = _scans(dev)
     f = tmp_path / "tuned.json"
     bp = BatchedProjector2D(scans, dev, tuner_file=str(f))
-    best, timing = bp.autotune(images, rays_per_item=(256, 1024), groups_per_cu=(1, 4), reps=1)
-    assert best in timing and len(timing) == 4 and f.exists()
+    best, timing = bp.autotune(images, rays_per_item=(256, 1024), groups_per_cu=(1, 4), rays_per_part=(512, 8192),
+                               reps=1)
+    assert best[:2] in timing["forward"] and len(timing["forward"]) == 4 and f.exists()
+    assert best[2] in timing["back"] and len(timing["back"]) == 2
     bp2 = BatchedProjector2D(scans, dev, tuner_file=str(f))
-    assert (bp2.rays_per_item, bp2.groups_per_cu) == best
+    assert (bp2.rays_per_item, bp2.groups_per_cu, bp2.rays_per_part) == best
     sinos = bp2.forward(images)
     assert all(torch.equal(s, _single(sc, im, dev)) for s, sc, im in zip(sinos, scans, images))
