@@ -6,7 +6,7 @@ batch is one persistent launch:
 
 - forward (``ctlab_forward_project_2d_batch``): workgroups take work items of
   ``rays_per_item`` consecutive rays from a per-batch queue, one ray per lane;
-- back (``ctlab_back_project_2d_batch_tiled``): work units are (64x64 image
+- back (``ctlab_back_project_2d_batch_tiled``): work units are (128x64 image
   tile, ``rays_per_part`` consecutive rays); a unit sums into LDS and writes
   its tile once, so no segment costs a global atomic.  ``tiled_back=False``
   selects the one-ray-per-lane atomic launch (``ctlab_back_project_2d_batch``)
@@ -41,7 +41,7 @@ SCAN_DTYPE = np.dtype([("image", "<u8"), ("sino", "<u8"), ("src", "<u8"), ("dst"
 assert SCAN_DTYPE.itemsize == 88
 
 N_CU = 256
-# Tiled back projection: workgroups of 1024 lanes with 41 KB of LDS, two per CU.
+# Tiled back projection: workgroups of 1024 lanes with 75 KB of LDS, two per CU.
 TILED_GROUPS_PER_CU = 2
 
 
@@ -77,6 +77,8 @@ class BatchedProjector2D:
                  tiled_back=True):
         self.scans = list(scans)
         self.device = torch.device("cuda") if device is None else torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:  # tensors always carry the index
+            self.device = torch.device("cuda", torch.cuda.current_device())
         for sc in self.scans:
             if sc.src.device != self.device:
                 sc.src, sc.dst = sc.src.to(self.device), sc.dst.to(self.device)
@@ -110,7 +112,7 @@ class BatchedProjector2D:
     # -- descriptor table -------------------------------------------------------
     def _build(self, images, sinos, rays_per_item, tiled=False):
         """Descriptor table and total work count: items of rays_per_item rays,
-        or (tiled) units of 64x64 tiles x rays_per_item rays."""
+        or (tiled) units of 128x64 tiles x rays_per_item rays."""
         lib = N.load() if tiled else None
         rec = np.zeros(len(self.scans), SCAN_DTYPE)
         begin = 0

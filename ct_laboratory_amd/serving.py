@@ -58,9 +58,11 @@ def _projector(req: MAPRequest, device):
     from .tomography import ConeBeam3DProjector, FanBeam2DProjector
 
     if req.kind == "fan2d":
-        # 1 mm pixels, 1 mm bins, SID 1000 / SDD 1500 (the fan-beam configuration)
+        # 1 mm pixels, 1 mm bins, SID 1000 / SDD 1500 (the fan-beam configuration).  On the
+        # fly: no crossing table to hold (4 (2n + 2) bytes per ray), and the back
+        # projection sums LDS tiles instead of streaming the table into global atomics.
         return FanBeam2DProjector(req.n, req.n, req.n_view, req.det, 1000.0, 1500.0, det_spacing=1.0,
-                                  pixel_size=1.0, device=device)
+                                  pixel_size=1.0, device=device, precomputed_intersections=False)
     # 1 mm voxels, 1.5 mm pixels, SID 1000 / SDD 1500 (the cone-beam configuration)
     return ConeBeam3DProjector(req.n, req.n, req.n, req.n_view, req.det, req.det, 1000.0, 1500.0, det_spacing=1.5,
                                voxel_size=1.0, device=device)
@@ -87,10 +89,10 @@ def build_problem(req: MAPRequest, device) -> None:
 def resident_bytes(req: MAPRequest) -> int:
     """Device bytes a request holds while active: rays (src + dst), measurements,
     projection and residual buffers, two iterates, back projection, preconditioner
-    and the geometry's crossing table (2D) or brick plan (3D)."""
+    and (3D) the brick plan."""
     if req.kind == "fan2d":
         n_ray, vox = req.n_view * req.det, req.n * req.n
-        geometry = 16 * n_ray + 4 * n_ray * (2 * req.n + 2)
+        geometry = 16 * n_ray
     else:
         n_ray, vox = req.n_view * req.det * req.det, req.n ** 3
         bricks = math.ceil(req.n / 32) ** 2 * math.ceil(req.n / 8)  # 32 x 32 x 8 bricks

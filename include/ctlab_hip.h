@@ -88,7 +88,12 @@ int ctlab_forward_project_2d_otf(const float* image, int64_t batch, int64_t n_ro
                                  const float* b, int mode, int zero_rule, float* out, ctlab_stream_t stream);
 
 /* Replaces _C.back_project_2d_on_the_fly_cuda (bindings.cpp:78-80;
- * src/ct_projector_2d.cu:442-465). */
+ * src/ct_projector_2d.cu:442-465).  out [batch, n_row, n_col]: zeroed first
+ * unless accumulate != 0.  Computed over 128x64 image tiles summed in LDS in
+ * 64-bit fixed point (no global atomic per segment); when the rays are split
+ * into several parts per tile (more than 4096 rays and few tiles), the parts
+ * combine with one float atomic add per pixel, so the last bits then depend on
+ * their order. */
 int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64_t n_ray, const float* src,
                               const float* dst, const float* M_inv, const float* b, int64_t n_row,
                               int64_t n_col, int mode, int zero_rule, int accumulate, float* out,
@@ -259,10 +264,10 @@ int ctlab_back_project_2d_batch(const ctlab_scan2d* scans, int64_t n_scan, int64
 
 /* Back projections of all scans without global atomics per segment (no
  * counterpart in the reference; same results as ctlab_back_project_2d_batch
- * within float summation order).  Work units are (64x64 image tile, range of
+ * within float summation order).  Work units are (128x64 image tile, range of
  * rays_per_part consecutive rays): in this call item_begin is the scan's first
- * UNIT, and scan k owns ceil(n_row/64) * ceil(n_col/64) * ceil(n_ray/rays_per_part)
- * units; total_units is their sum.  A unit sums its rays' contributions to its
+ * UNIT, and scan k owns ceil(n_row/128) * ceil(n_col/64) * ceil(n_ray/rays_per_part)
+ * units (ctlab_back_project_2d_tile_units); total_units is their sum.  A unit sums its rays' contributions to its
  * tile in LDS (64-bit fixed point, order independent) and writes the tile
  * once: with plain stores when the scan has one part, else with one float
  * atomic add per pixel.  Images are zeroed first unless accumulate != 0.

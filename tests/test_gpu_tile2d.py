@@ -48,7 +48,7 @@ def _back(dev, y, src, dst, minv, b, shape, mode, zr, accumulate=0, out=None):
 @pytest.mark.parametrize("mname,mode", MODES)
 def test_tiles_vs_oracle(dev, mname, mode, n_ray):
     rng = np.random.default_rng(700 + n_ray + mode)
-    shape = (150, 97)  # 3 x 2 tiles, ragged in both directions
+    shape = (150, 97)  # 2 x 2 tiles of 128 x 64, ragged in both directions
     minv, b, s = _geometry(rng, shape, rotated=(n_ray == 20000))
     src, dst = random_rays(rng, shape, s, n_ray, "far")
     zr = N.ZERO_EXACT if mname == "torch" else N.ZERO_THRESH
@@ -114,7 +114,7 @@ def test_tiles_zero_and_empty(dev):
                                              _t(b, dev), *shape)
     assert out.shape == (1,) + shape and float(out.abs().sum()) == 0.0
     # one ray along pixel row 40 (through the pixel centres): length 1 in each of
-    # its 80 pixels, nothing elsewhere (tiles 0-1 and 2-3 of the 2 x 2 grid)
+    # its 80 pixels, nothing elsewhere (across both tiles of the 1 x 2 grid)
     got = _back(dev, np.ones((1, 1), np.float32), np.array([[0.5, -100.0]], np.float32),
                 np.array([[0.5, 100.0]], np.float32), minv, b, shape, N.MODE_TORCH, N.ZERO_EXACT).cpu().numpy()[0]
     # (the torch rule measures lengths between float world points near |y| = 100,
