@@ -1,6 +1,6 @@
 """GPU tests of the continuous-batching MAP server: each result is the same alone
 and in the mix (cone-beam requests bit for bit: their kernels are deterministic;
-fan-beam requests within 1e-5, their table back projector uses float atomics),
+fan-beam requests within 1e-5, the parts of a back-projection tile combine with float atomics),
 and the same as the plain per-request autograd loop within 1e-4 (same iterations;
 autograd forms the Gaussian and smoothness gradients with other float
 operations than the fused kernels)."""
