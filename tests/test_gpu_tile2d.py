@@ -67,6 +67,30 @@ def test_tiles_vs_oracle(dev, mname, mode, n_ray):
         assert np.array_equal(got, _back(dev, y, src, dst, minv, b, shape, mode, zr).cpu().numpy())
 
 
+@pytest.mark.parametrize("kind", ["inside", "distant", "lattice"])
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_tiles_ray_kinds(dev, mname, mode, kind):
+    """Rays that start or end inside the image, rays from ~4e4 pixels away
+    (midpoint error bounds above one half: the exhaustive walk and the widest
+    filter margins), and half-integer lattice rays on the identity grid (rays
+    along tile and pixel boundaries, through corners, axis-parallel)."""
+    rng = np.random.default_rng(900 + 7 * mode + len(kind))
+    shape = (200, 150)  # 2 x 3 tiles
+    if kind == "lattice":
+        minv, b, s = np.eye(2, dtype=np.float32), (-(np.array(shape) - 1) / 2).astype(np.float32), np.ones(2)
+    else:
+        minv, b, s = _geometry(rng, shape, rotated=True)
+    src, dst = random_rays(rng, shape, s, 6000, kind)
+    zr = N.ZERO_EXACT if mname == "torch" else N.ZERO_THRESH
+    ozr = "exact" if zr == N.ZERO_EXACT else "thresh"
+    y = rng.standard_normal((1, src.shape[0])).astype(np.float32)
+    got = _back(dev, y, src, dst, minv, b, shape, mode, zr).cpu().numpy()
+    ref = O.back(y, shape, minv, b, src, dst, mode=mname, zero_rule=ozr)
+    bound = O.back(np.abs(y), shape, minv, b, src, dst, mode=mname, zero_rule=ozr)
+    assert float(bound.max()) > 0.0
+    assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-10 * float(np.abs(y).max()))
+
+
 @pytest.mark.parametrize("mname,mode", MODES)
 def test_tiles_scales_and_nonfinite(dev, mname, mode):
     rng = np.random.default_rng(77 + mode)
