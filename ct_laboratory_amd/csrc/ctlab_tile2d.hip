@@ -7,9 +7,9 @@
 // independent of the order of the rays, as in the 3D brick back projector)
 // and then writes the tile once.  Per chunk of 4096 rays, a conservative slab
 // test against the tile widened by the ray's rounding bound plus one pixel
-// keeps the rays that can reach it, compacted in ray order into an LDS queue
-// so that a wave's lanes trace neighbouring rays of similar length; each kept
-// ray is traced with the exact traversal clipped to the tile
+// keeps the rays that can reach it, filed by estimated length into an LDS
+// queue so that a wave's lanes trace rays of similar length; each kept ray is
+// traced with the exact traversal clipped to the tile
 // (trace_ray_window), so the tile receives exactly the segments whose pixel it
 // contains.  Global float atomics, which execute past L2 and bound the
 // one-ray-per-lane back projection, are left only for combining parts: a tile
@@ -33,11 +33,19 @@ constexpr int kTileChunk = kTilePerThread * kTileThreads;  // rays filtered per 
 
 static_assert(sizeof(ctlab_scan2d) == 88, "ctlab_scan2d layout");
 
+// Rays of a chunk are traced in order of estimated crossing count, filed into
+// quarter-octave buckets, so that a wave's lanes trace rays of similar length
+// (in ray order a wave can span a tile's whole shadow, from corner-clipping to
+// full-width rays).
+constexpr bool kSortByLength = true;
+constexpr int kTileBuckets = 32;
+
 struct TileLds {
     unsigned long long acc[kTileCells];
     uint16_t queue[kTileChunk];
     float red[kTileThreads / 64];
     uint32_t count;
+    uint32_t bcnt[kTileBuckets + 1];
 };
 
 // Longest segment any ray can have inside one pixel: the longer diagonal of
@@ -59,9 +67,11 @@ __device__ __forceinline__ double pixel_diagonal(const Grid<2>& g) {
 // E + 1: the extra half pixel covers this test's own float rounding (relative
 // 2^-22 of coordinates below 2^20 pixels).  It drops rays only for E > 8.5, so
 // a rounding difference from the traversal's own E near 8 keeps the ray.
+// Returns 0 for a ray that cannot reach the tile, else an estimate (>= 1) of
+// its crossings in the widened tile.
 template <int MODE, bool DIAG>
-__device__ __forceinline__ bool tile_may_hit(const Grid<2>& g, const Window<2>& win, const float* s,
-                                             const float* d) {
+__device__ __forceinline__ uint32_t tile_may_hit(const Grid<2>& g, const Window<2>& win, const float* s,
+                                                 const float* d) {
     float sv[2], dv[2];
     to_voxel_t<2, MODE, DIAG>(g, s, sv);
     to_voxel_t<2, MODE, DIAG>(g, d, dv);
@@ -72,18 +82,20 @@ __device__ __forceinline__ bool tile_may_hit(const Grid<2>& g, const Window<2>& 
 #pragma unroll
         for (int j = 0; j < 2; ++j) mag += fabsf(g.minv[a * 2 + j]) * (fabsf(s[j]) + fabsf(d[j]) + fabsf(g.b[j]));
         const float m = mag * 7.62939453125e-06f + 1.0f;  // E + 1 pixel, E = mag * 2^-17
-        if (!(m <= 9.5f)) return false;  // E well above the traversal's limit of 8, or NaN: no segments
+        if (!(m <= 9.5f)) return 0u;  // E well above the traversal's limit of 8, or NaN: no segments
         const float lo = (float)win.lo[a] - 0.5f - m, hi = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
         const float dir = dv[a] - sv[a];
         if (fabsf(dir) < 1.0e-20f) {
-            if (!(sv[a] >= lo && sv[a] <= hi)) return false;
+            if (!(sv[a] >= lo && sv[a] <= hi)) return 0u;
             continue;
         }
         const float ta = (lo - sv[a]) / dir, tb = (hi - sv[a]) / dir;
         t0 = fmaxf(t0, fminf(ta, tb));
         t1 = fminf(t1, fmaxf(ta, tb));
     }
-    return t0 <= t1;
+    if (!(t0 <= t1)) return 0u;
+    const float cross = (t1 - t0) * (fabsf(dv[0] - sv[0]) + fabsf(dv[1] - sv[1]));
+    return (uint32_t)fminf(cross, 65535.0f) + 1u;
 }
 
 // Block-wide maximum (all kTileThreads threads call it).
@@ -141,27 +153,62 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     // 3. filter, queue and trace, chunk by chunk
     if (vmax != 0.0f) {
         const float s1 = fs.s1, s2 = fs.s2;
-        const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
         for (int64_t c0 = r0; c0 < r1; c0 += kTileChunk) {
             if (tid == 0) L.count = 0u;
+            if (kSortByLength && tid <= kTileBuckets) L.bcnt[tid] = 0u;
             __syncthreads();
+            if constexpr (kSortByLength) {
+                uint32_t key[kTilePerThread];
 #pragma unroll
-            for (int k = 0; k < kTilePerThread; ++k) {
-                const int off = tid + k * kTileThreads;
-                const int64_t r = c0 + off;
-                bool hit = false;
-                if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
-                    const float s[2] = {src[2 * r], src[2 * r + 1]};
-                    const float d[2] = {dst[2 * r], dst[2 * r + 1]};
-                    hit = tile_may_hit<MODE, DIAG>(g, win, s, d);
+                for (int k = 0; k < kTilePerThread; ++k) {
+                    const int off = tid + k * kTileThreads;
+                    const int64_t r = c0 + off;
+                    key[k] = ~0u;
+                    if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
+                        const float s[2] = {src[2 * r], src[2 * r + 1]};
+                        const float d[2] = {dst[2 * r], dst[2 * r + 1]};
+                        const uint32_t est = tile_may_hit<MODE, DIAG>(g, win, s, d);
+                        if (est) {
+                            const uint32_t l2 = 31u - (uint32_t)__clz(est);  // est in [2^l2, 2^(l2+1))
+                            const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kTileBuckets - 1));
+                            key[k] = (bk << 16) | atomicAdd(&L.bcnt[bk], 1u);
+                        }
+                    }
                 }
-                const uint64_t mask = __ballot(hit);
-                const uint32_t before =
-                    __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                uint32_t base = 0u;
-                if (lane == 0 && mask != 0ull) base = atomicAdd(&L.count, (uint32_t)__popcll(mask));
-                base = __shfl(base, 0);
-                if (hit) L.queue[base + before] = (uint16_t)off;
+                __syncthreads();
+                if (tid == 0) {
+                    uint32_t run = 0u;
+                    for (int k = 0; k < kTileBuckets; ++k) {
+                        const uint32_t c = L.bcnt[k];
+                        L.bcnt[k] = run;
+                        run += c;
+                    }
+                    L.count = run;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kTilePerThread; ++k)
+                    if (key[k] != ~0u) L.queue[L.bcnt[key[k] >> 16] + (key[k] & 0xffffu)] = (uint16_t)(tid + k * kTileThreads);
+            } else {
+                const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+#pragma unroll
+                for (int k = 0; k < kTilePerThread; ++k) {
+                    const int off = tid + k * kTileThreads;
+                    const int64_t r = c0 + off;
+                    bool hit = false;
+                    if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
+                        const float s[2] = {src[2 * r], src[2 * r + 1]};
+                        const float d[2] = {dst[2 * r], dst[2 * r + 1]};
+                        hit = tile_may_hit<MODE, DIAG>(g, win, s, d) != 0u;
+                    }
+                    const uint64_t mask = __ballot(hit);
+                    const uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    uint32_t base = 0u;
+                    if (lane == 0 && mask != 0ull) base = atomicAdd(&L.count, (uint32_t)__popcll(mask));
+                    base = __shfl(base, 0);
+                    if (hit) L.queue[base + before] = (uint16_t)off;
+                }
             }
             __syncthreads();
             const uint32_t hits = L.count;
