@@ -37,7 +37,6 @@ static_assert(sizeof(ctlab_scan2d) == 88, "ctlab_scan2d layout");
 // quarter-octave buckets, so that a wave's lanes trace rays of similar length
 // (in ray order a wave can span a tile's whole shadow, from corner-clipping to
 // full-width rays).
-constexpr bool kSortByLength = true;
 constexpr int kTileBuckets = 32;
 
 struct TileLds {
@@ -154,62 +153,40 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     if (vmax != 0.0f) {
         const float s1 = fs.s1, s2 = fs.s2;
         for (int64_t c0 = r0; c0 < r1; c0 += kTileChunk) {
-            if (tid == 0) L.count = 0u;
-            if (kSortByLength && tid <= kTileBuckets) L.bcnt[tid] = 0u;
+            if (tid <= kTileBuckets) L.bcnt[tid] = 0u;
             __syncthreads();
-            if constexpr (kSortByLength) {
-                uint32_t key[kTilePerThread];
+            uint32_t key[kTilePerThread];
 #pragma unroll
-                for (int k = 0; k < kTilePerThread; ++k) {
-                    const int off = tid + k * kTileThreads;
-                    const int64_t r = c0 + off;
-                    key[k] = ~0u;
-                    if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
-                        const float s[2] = {src[2 * r], src[2 * r + 1]};
-                        const float d[2] = {dst[2 * r], dst[2 * r + 1]};
-                        const uint32_t est = tile_may_hit<MODE, DIAG>(g, win, s, d);
-                        if (est) {
-                            const uint32_t l2 = 31u - (uint32_t)__clz(est);  // est in [2^l2, 2^(l2+1))
-                            const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kTileBuckets - 1));
-                            key[k] = (bk << 16) | atomicAdd(&L.bcnt[bk], 1u);
-                        }
+            for (int k = 0; k < kTilePerThread; ++k) {
+                const int off = tid + k * kTileThreads;
+                const int64_t r = c0 + off;
+                key[k] = ~0u;
+                if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
+                    const float s[2] = {src[2 * r], src[2 * r + 1]};
+                    const float d[2] = {dst[2 * r], dst[2 * r + 1]};
+                    const uint32_t est = tile_may_hit<MODE, DIAG>(g, win, s, d);
+                    if (est) {
+                        const uint32_t l2 = 31u - (uint32_t)__clz(est);  // est in [2^l2, 2^(l2+1))
+                        const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kTileBuckets - 1));
+                        key[k] = (bk << 16) | atomicAdd(&L.bcnt[bk], 1u);
                     }
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    uint32_t run = 0u;
-                    for (int k = 0; k < kTileBuckets; ++k) {
-                        const uint32_t c = L.bcnt[k];
-                        L.bcnt[k] = run;
-                        run += c;
-                    }
-                    L.count = run;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < kTilePerThread; ++k)
-                    if (key[k] != ~0u) L.queue[L.bcnt[key[k] >> 16] + (key[k] & 0xffffu)] = (uint16_t)(tid + k * kTileThreads);
-            } else {
-                const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-#pragma unroll
-                for (int k = 0; k < kTilePerThread; ++k) {
-                    const int off = tid + k * kTileThreads;
-                    const int64_t r = c0 + off;
-                    bool hit = false;
-                    if (r < r1 && sn[r] != 0.0f) {  // a zero value contributes exactly zero
-                        const float s[2] = {src[2 * r], src[2 * r + 1]};
-                        const float d[2] = {dst[2 * r], dst[2 * r + 1]};
-                        hit = tile_may_hit<MODE, DIAG>(g, win, s, d) != 0u;
-                    }
-                    const uint64_t mask = __ballot(hit);
-                    const uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                    uint32_t base = 0u;
-                    if (lane == 0 && mask != 0ull) base = atomicAdd(&L.count, (uint32_t)__popcll(mask));
-                    base = __shfl(base, 0);
-                    if (hit) L.queue[base + before] = (uint16_t)off;
                 }
             }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t run = 0u;
+                for (int k = 0; k < kTileBuckets; ++k) {
+                    const uint32_t c = L.bcnt[k];
+                    L.bcnt[k] = run;
+                    run += c;
+                }
+                L.count = run;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kTilePerThread; ++k)
+                if (key[k] != ~0u)
+                    L.queue[L.bcnt[key[k] >> 16] + (key[k] & 0xffffu)] = (uint16_t)(tid + k * kTileThreads);
             __syncthreads();
             const uint32_t hits = L.count;
             for (uint32_t q = tid; q < hits; q += kTileThreads) {
