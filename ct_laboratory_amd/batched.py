@@ -40,7 +40,7 @@ SCAN_DTYPE = np.dtype([("image", "<u8"), ("sino", "<u8"), ("src", "<u8"), ("dst"
                        ("zero_rule", "<i4"), ("item_begin", "<i8")])
 assert SCAN_DTYPE.itemsize == 88
 
-N_CU = 256
+N_CU = 256  # MI355X; the projector reads the device's own count
 # Tiled back projection: workgroups of 1024 lanes with 75 KB of LDS, two per CU.
 TILED_GROUPS_PER_CU = 2
 
@@ -82,6 +82,7 @@ class BatchedProjector2D:
         for sc in self.scans:
             if sc.src.device != self.device:
                 sc.src, sc.dst = sc.src.to(self.device), sc.dst.to(self.device)
+        self.n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count or N_CU
         self.counter = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.sinos = [torch.empty(sc.n_ray, device=self.device) for sc in self.scans]
         self.images = [torch.empty(sc.n_row, sc.n_col, device=self.device) for sc in self.scans]
@@ -152,13 +153,13 @@ class BatchedProjector2D:
         with torch.cuda.device(self.device):
             if tiled:
                 N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(self.scans), items, rays_per_item,
-                       TILED_GROUPS_PER_CU * N_CU, N.ptr(self.counter), int(bool(accumulate)), stream)
+                       TILED_GROUPS_PER_CU * self.n_cu, N.ptr(self.counter), int(bool(accumulate)), stream)
             elif back:
                 N.call("ctlab_back_project_2d_batch", N.ptr(table), len(self.scans), items, rays_per_item,
-                       groups_per_cu * N_CU, N.ptr(self.counter), int(bool(accumulate)), stream)
+                       groups_per_cu * self.n_cu, N.ptr(self.counter), int(bool(accumulate)), stream)
             else:
                 N.call("ctlab_forward_project_2d_batch", N.ptr(table), len(self.scans), items,
-                       rays_per_item, groups_per_cu * N_CU, N.ptr(self.counter), stream)
+                       rays_per_item, groups_per_cu * self.n_cu, N.ptr(self.counter), stream)
 
     # -- projections ------------------------------------------------------------
     def forward(self, images):
