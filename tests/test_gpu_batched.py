@@ -84,3 +84,25 @@ def test_autotune_and_cache(dev, tmp_path):
     assert (bp2.rays_per_item, bp2.groups_per_cu, bp2.rays_per_part) == best
     sinos = bp2.forward(images)
     assert all(torch.equal(s, _single(sc, im, dev)) for s, sc, im in zip(sinos, scans, images))
+
+
+def test_tiled_batch_accumulate_and_spare_units(dev):
+    """The tiled multi-scan back projection called through the C ABI: accumulate
+    adds to the images, and a table whose total counts spare units past the
+    last scan gives the same images (the spare units are skipped)."""
+    scans, images = _scans(dev)
+    bp = BatchedProjector2D(scans, dev, rays_per_part=512)
+    ys = [torch.from_numpy(np.random.default_rng(k).uniform(0, 1, sc.n_ray).astype(np.float32)).to(dev)
+          for k, sc in enumerate(scans)]
+    ref = [b.clone() for b in bp.back(ys)]
+    table, units = bp._build(bp.images, ys, 512, tiled=True)
+    base = [torch.full_like(im, 0.25) for im in bp.images]
+    for k in range(len(scans)):
+        bp.images[k].copy_(base[k])
+    stream = N.stream_of(bp.counter)
+    N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(scans), units + 37, 512, 2 * bp.n_cu,
+           N.ptr(bp.counter), 1, stream)
+    torch.cuda.synchronize()
+    for k in range(len(scans)):
+        got = bp.images[k].cpu().numpy()
+        assert rel_err(got, base[k].cpu().numpy() + ref[k].cpu().numpy()) < 1e-5
