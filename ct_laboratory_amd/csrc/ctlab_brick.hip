@@ -339,6 +339,28 @@ constexpr int kChunk = kPerThread * kBrickThreads;  // queue entries are chunk o
 constexpr int kBuckets = 32;  // quarter-octave length classes (1, 1.25, 1.5, 1.75, 2, 2.5, ... crossings)
 constexpr int kStage = 64;
 
+// local / w and local % w of a decoded item.  For local < 2^22 a float
+// reciprocal gives a quotient estimate within 1/2 of local / w (relative error
+// below 2^-22; exact for w = 1), so its truncation is off by at most one, fixed
+// by one correction of the remainder; larger values use integer division.
+__device__ __forceinline__ uint32_t div_rect(uint32_t a, uint32_t b, uint32_t* rem) {
+    if (a < (1u << 22)) {
+        uint32_t q = (uint32_t)((float)a * __builtin_amdgcn_rcpf((float)b));
+        int32_t r = (int32_t)(a - q * b);
+        if (r < 0) {
+            --q;
+            r += (int32_t)b;
+        } else if (r >= (int32_t)b) {
+            ++q;
+            r -= (int32_t)b;
+        }
+        *rem = (uint32_t)r;
+        return q;
+    }
+    *rem = a % b;
+    return a / b;
+}
+
 // Largest j in [0, n) with prefix(j) <= item, for prefixes non-decreasing in j
 // and prefix(0) <= item.
 __device__ __forceinline__ int staged_view(const int4* s_plan, int n, uint32_t item) {
@@ -405,8 +427,9 @@ __device__ __forceinline__ void brick_items(const Grid<3>& g, const Window<3>& w
             j = staged_view(s_plan, kStage, item);
             const int4 e = s_plan[j];
             const uint32_t local = item - (uint32_t)e.w, w = (uint32_t)e.z;
-            r = e.x + (int)(local / w);
-            c = e.y + (int)(local % w);
+            uint32_t rem;
+            r = e.x + (int)div_rect(local, w, &rem);
+            c = e.y + (int)rem;
             return (uint32_t)(((int64_t)(vs + j) * rows + r) * cols + c);
         };
         uint32_t key[kPerThread];
