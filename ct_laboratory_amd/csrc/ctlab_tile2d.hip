@@ -123,10 +123,9 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     win.ext[1] = min(kTileC, g.n[1] - win.lo[1]);
     win.stride[0] = kTileStride;
     win.stride[1] = 1;
-    const int ec = win.ext[1], npx = win.ext[0] * ec;
-    auto gidx = [&](int i) -> int64_t {
-        return (int64_t)(win.lo[0] + i / ec) * g.n[1] + (win.lo[1] + i % ec);
-    };
+    // image index of tile pixel (r, c); pixels are enumerated as p = r * kTileC + c
+    // over the full tile (shifts, not divisions), skipping those past a ragged edge
+    auto gidx = [&](int r, int c) -> int64_t { return (int64_t)(win.lo[0] + r) * g.n[1] + (win.lo[1] + c); };
 
     for (int i = tid; i < kTileCells; i += kTileThreads) L.acc[i] = 0ull;
 
@@ -145,7 +144,8 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     const FixedScale fs = fixed_scale(2.0 * (double)vmax * pixel_diagonal(g), (double)(r1 - r0), g.finite);
 
     if (!fs.fixed && direct && !accumulate) {  // float atomics below need a zeroed tile
-        for (int i = tid; i < npx; i += kTileThreads) img[gidx(i)] = 0.0f;
+        for (int p = tid; p < kTileR * kTileC; p += kTileThreads)
+            if (p / kTileC < win.ext[0] && p % kTileC < win.ext[1]) img[gidx(p / kTileC, p % kTileC)] = 0.0f;
     }
     __syncthreads();
 
@@ -200,8 +200,7 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
                 } else {
                     const float val = sn[r];
                     trace_ray_window<2, MODE, DIAG, false>(g, win, s, d, [&](uint32_t li, float len) {
-                        const int i = (int)(li / kTileStride) * ec + (int)(li % kTileStride);
-                        unsafeAtomicAdd(img + gidx(i), val * len);
+                        unsafeAtomicAdd(img + gidx((int)(li / kTileStride), (int)(li % kTileStride)), val * len);
                     });
                 }
             }
@@ -212,9 +211,11 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     // 4. write the tile (rows of the image, coalesced)
     if (!fs.fixed) return;
     const double inv = ldexp(1.0, -fs.F);
-    for (int i = tid; i < npx; i += kTileThreads) {
-        const float v = (float)((double)(long long)L.acc[(i / ec) * kTileStride + i % ec] * inv);
-        float* o = img + gidx(i);
+    for (int p = tid; p < kTileR * kTileC; p += kTileThreads) {
+        const int r = p / kTileC, c = p % kTileC;
+        if (r >= win.ext[0] || c >= win.ext[1]) continue;
+        const float v = (float)((double)(long long)L.acc[r * kTileStride + c] * inv);
+        float* o = img + gidx(r, c);
         if (direct)
             *o = accumulate ? *o + v : v;
         else if (v != 0.0f)
