@@ -11,7 +11,7 @@ import torch
 import oracle as O
 from ct_laboratory_amd import _C
 from ct_laboratory_amd import _native as N
-from helpers import random_rays
+from helpers import random_rays, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -146,3 +146,28 @@ def test_tiles_zero_and_empty(dev):
     np.testing.assert_allclose(got[40], 1.0, rtol=0, atol=2 * 2.0 ** -16)
     assert abs(float(got[40].astype(np.float64).sum()) - 80.0) < 1e-4
     assert float(np.abs(np.delete(got, 40, axis=0)).sum()) == 0.0
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_tiles_nonfinite_geometry(dev, mname, mode):
+    """Rays with INF or NaN end points contribute nothing (every world point of
+    such a ray is INF or NaN and masked), the other rays as usual; a map with a
+    NaN entry gives a zero image (every index is out of range)."""
+    rng = np.random.default_rng(31 + mode)
+    shape = (130, 70)
+    minv, b, s = _geometry(rng, shape, rotated=True)
+    src, dst = random_rays(rng, shape, s, 3000, "far")
+    src[::7, 0] = np.inf
+    dst[3::11, 1] = np.nan
+    src[5::13, 1] = -np.inf
+    zr = N.ZERO_EXACT if mname == "torch" else N.ZERO_THRESH
+    ozr = "exact" if zr == N.ZERO_EXACT else "thresh"
+    y = rng.uniform(0.5, 1.5, (1, src.shape[0])).astype(np.float32)
+    got = _back(dev, y, src, dst, minv, b, shape, mode, zr).cpu().numpy()
+    ref = O.back(y, shape, minv, b, src, dst, mode=mname, zero_rule=ozr)
+    assert np.isfinite(got).all() and float(ref.max()) > 0.0
+    assert rel_err(got, ref) < 1e-5
+    bad = minv.copy()
+    bad[0, 1] = np.nan
+    out = _back(dev, y, src, dst, bad, b, shape, mode, zr, out=torch.full((1,) + shape, 3.0, device=dev))
+    assert float(out.abs().sum()) == 0.0
