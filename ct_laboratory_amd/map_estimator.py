@@ -145,8 +145,10 @@ class ProjectorOps:
                 n_view = self.n_ray // (self.det_shape[0] * self.det_shape[1])
                 self.ws = torch.empty(n_view, device=self.device)
         else:
-            self.table = proj.tvals
-            self.zero_rule = N.ZERO_EXACT if proj.semantics == "torch" else N.ZERO_THRESH
+            # the module's own untouched table runs on the fly with the table's rule
+            self.table, sem, zr = proj._rule()
+            self.mode = semantics_mode(sem)
+            self.zero_rule = (N.ZERO_EXACT if sem == "torch" else N.ZERO_THRESH) if zr is None else zr
 
     def forward(self, x, out):
         if self.nd == 3:

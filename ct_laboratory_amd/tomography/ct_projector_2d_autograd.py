@@ -17,19 +17,22 @@ __all__ = ["CTProjector2DFunction"]
 
 
 class CTProjector2DFunction(torch.autograd.Function):
-    """forward(image [R,C] or [B,R,C], tvals, M, b, src, dst, backend='cuda', semantics='torch')
-    -> sinogram [n_ray] or [B, n_ray].  ``tvals=None`` uses the on-the-fly
-    traversal with the given segment rule."""
+    """forward(image [R,C] or [B,R,C], tvals, M, b, src, dst, backend='cuda', semantics='torch',
+    zero_rule=None) -> sinogram [n_ray] or [B, n_ray].  ``tvals=None`` uses the
+    on-the-fly traversal with the given segment rule (and zero rule, None: the
+    rule's own)."""
 
     @staticmethod
-    def forward(ctx, image, tvals, M, b, src, dst, backend="cuda", semantics="torch"):
+    def forward(ctx, image, tvals, M, b, src, dst, backend="cuda", semantics="torch", zero_rule=None):
         check_backend(backend)
         ctx.image_shape = image.shape
         ctx.semantics = semantics
+        ctx.zero_rule = zero_rule
         ctx.on_the_fly = tvals is None
         if ctx.on_the_fly:
             ctx.save_for_backward(M, b, src, dst)
-            return forward_project_2d_on_the_fly_hip(image.contiguous(), M, b, src, dst, semantics=semantics)
+            return forward_project_2d_on_the_fly_hip(image.contiguous(), M, b, src, dst, semantics=semantics,
+                                                     zero_rule=zero_rule)
         ctx.save_for_backward(tvals, M, b, src, dst)
         return forward_project_2d_hip(image.contiguous(), tvals, M, b, src, dst)
 
@@ -40,8 +43,8 @@ class CTProjector2DFunction(torch.autograd.Function):
         if ctx.on_the_fly:
             M, b, src, dst = ctx.saved_tensors
             grad = back_project_2d_on_the_fly_hip(grad_output, M, b, src, dst, n_row, n_col,
-                                                  semantics=ctx.semantics)
+                                                  semantics=ctx.semantics, zero_rule=ctx.zero_rule)
         else:
             tvals, M, b, src, dst = ctx.saved_tensors
             grad = back_project_2d_hip(grad_output, tvals, M, b, src, dst, n_row, n_col)
-        return grad, None, None, None, None, None, None, None
+        return grad, None, None, None, None, None, None, None, None

@@ -6,10 +6,23 @@ differentiable (CTProjector2DFunction); ``forward_project`` / ``back_project``
 are the raw operators.  ``precomputed_intersections=False`` uses the on-the-fly
 traversal instead of a table (no table memory; same segments under the
 ``"torch"`` rule).
+
+While the table is the one the module computed and nothing has written to it,
+the projections run on the fly with the table's own segment rule (torch
+crossings, midpoints and lengths; axes with |dir| < 1e-12 dropped, as
+compute_intersections_2d drops them): the traversal yields exactly the table's
+segments, in the same order (the forward is bit-identical), without reading
+4 (n_row + n_col + 2) bytes of table per ray, and the back projection sums LDS
+tiles instead of adding each segment with a global atomic.  A table that is
+replaced or written in place is consumed as given.
 """
 from __future__ import annotations
 
+import weakref
+
 import torch
+
+from .. import _native as N
 
 from ._backend import check_backend
 from .ct_projector_2d_autograd import CTProjector2DFunction
@@ -36,21 +49,34 @@ class CTProjector2DModule(torch.nn.Module):
         if precomputed_intersections:
             tvals = compute_intersections_2d_hip(self.n_row, self.n_col, self.M, self.b, self.src, self.dst)
         self.register_buffer("tvals", tvals)
+        self._own_table = None if tvals is None else (weakref.ref(tvals), tvals._version)
+
+    def _rule(self):
+        """(tvals, semantics, zero_rule) to project with: no table and the
+        table's rule while the table is the module's own, untouched one."""
+        t = self.tvals
+        if t is None:
+            return None, self.semantics, None
+        if self._own_table is not None and self._own_table[0]() is t and t._version == self._own_table[1]:
+            return None, "torch", N.ZERO_THRESH
+        return t, self.semantics, None
 
     def forward_project(self, image):
-        if self.tvals is None:
+        tvals, sem, zr = self._rule()
+        if tvals is None:
             return forward_project_2d_on_the_fly_hip(image.contiguous(), self.M, self.b, self.src, self.dst,
-                                                     semantics=self.semantics)
-        return forward_project_2d_hip(image.contiguous(), self.tvals, self.M, self.b, self.src, self.dst)
+                                                     semantics=sem, zero_rule=zr)
+        return forward_project_2d_hip(image.contiguous(), tvals, self.M, self.b, self.src, self.dst)
 
     def back_project(self, sinogram):
-        if self.tvals is None:
+        tvals, sem, zr = self._rule()
+        if tvals is None:
             return back_project_2d_on_the_fly_hip(sinogram.contiguous(), self.M, self.b, self.src, self.dst,
-                                                  self.n_row, self.n_col, semantics=self.semantics)
-        return back_project_2d_hip(sinogram.contiguous(), self.tvals, self.M, self.b, self.src, self.dst,
+                                                  self.n_row, self.n_col, semantics=sem, zero_rule=zr)
+        return back_project_2d_hip(sinogram.contiguous(), tvals, self.M, self.b, self.src, self.dst,
                                    self.n_row, self.n_col)
 
     def forward(self, image):
         """image [R,C] or [B,R,C] -> sinogram [n_ray] or [B,n_ray] (differentiable)."""
-        return CTProjector2DFunction.apply(image, self.tvals, self.M, self.b, self.src, self.dst, self.backend,
-                                           self.semantics)
+        tvals, sem, zr = self._rule()
+        return CTProjector2DFunction.apply(image, tvals, self.M, self.b, self.src, self.dst, self.backend, sem, zr)

@@ -75,6 +75,33 @@ def test_fanbeam_module_matches_oracle(dev, tables):
     assert out2.shape == (2, src.shape[0]) and rel_err(out2[1], 2 * ref) < 1e-5
 
 
+def test_own_table_runs_on_the_fly(dev):
+    """A 2D module projecting with the table it computed runs the on-the-fly
+    kernels with the table's rule: the forward equals the table consumer bit for
+    bit, the back projection to 1e-5; once the table is written in place (or
+    replaced) the module consumes it as given."""
+    from ct_laboratory_amd.tomography.ct_projector_2d_hip import back_project_2d_hip, forward_project_2d_hip
+
+    n = 80  # 1 x 2 tiles of 128 x 64
+    proj = FanBeam2DProjector(n, n, 45, 120, 300.0, 450.0, det_spacing=1.0, pixel_size=1.0, device=dev)
+    img = torch.from_numpy(ellipse_phantom_2d(n, n, 1.0, seed=4)).to(dev)
+    assert proj._rule()[0] is None
+    table_fwd = forward_project_2d_hip(img, proj.tvals, proj.M, proj.b, proj.src, proj.dst)
+    assert torch.equal(proj.forward_project(img), table_fwd)
+    assert torch.equal(proj(img), table_fwd)
+    y = torch.from_numpy(np.random.default_rng(9).uniform(0, 1, proj.src.shape[0]).astype(np.float32)).to(dev)
+    table_back = back_project_2d_hip(y, proj.tvals, proj.M, proj.b, proj.src, proj.dst, n, n)
+    assert rel_err(_np(proj.back_project(y)), _np(table_back)) < 1e-5
+    # an edited table is honoured: ray 0 loses its crossings
+    k = int(torch.nonzero(table_fwd > 0)[0])
+    with torch.no_grad():
+        proj.tvals[k] = float("inf")
+    assert proj._rule()[0] is proj.tvals
+    edited = proj.forward_project(img)
+    assert float(edited[k]) == 0.0
+    assert torch.equal(edited, forward_project_2d_hip(img, proj.tvals, proj.M, proj.b, proj.src, proj.dst))
+
+
 def test_autograd_2d_gradient_is_back_projection(dev):
     n = 48
     proj = FanBeam2DProjector(n, n, 40, 72, 300.0, 450.0, device=dev)
