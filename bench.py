@@ -92,6 +92,67 @@ def cpu_baseline(src, dst, vol, minv, b, mode_name, n_sample_views=8):
     }
 
 
+def fan2d_bench(a, dev) -> int:
+    """The 2D fan-beam configuration: 256x256 image at 1 mm, 360 views, 512 bins at
+    1 mm, SID 1000 / SDD 1500 (184,320 rays), seeded 10-ellipse phantom with a
+    0.02/mm background; FanBeam2DProjector with its crossing table as the
+    reference builds it (projected on the fly with the table's rule).  Forward +
+    back per step on the GPU, and the oracle (C restatement of the torch path) on
+    the full problem on the host cores (one warm-up, median of three).  Prints one
+    JSON line."""
+    import torch
+
+    import oracle as O
+    from ct_laboratory_amd.phantoms import ellipse_phantom_2d
+    from ct_laboratory_amd.tomography import FanBeam2DProjector
+
+    n, n_view, n_det = 256, 360, 512
+    proj = FanBeam2DProjector(n, n, n_view, n_det, 1000.0, 1500.0, det_spacing=1.0, pixel_size=1.0, device=dev)
+    img = torch.from_numpy(ellipse_phantom_2d(n, n, 1.0, seed=0)).to(dev)
+    n_ray = int(proj.src.shape[0])
+
+    def step():
+        proj.back_project(proj.forward_project(img))
+
+    for _ in range(max(1, a.warmup)):
+        step()
+    torch.cuda.synchronize()
+    steps = max(a.steps, 20)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t0) / steps
+
+    minv = torch.inverse(proj.M).to(torch.float32).cpu().numpy()
+    b, src, dst = proj.b.cpu().numpy(), proj.src.cpu().numpy(), proj.dst.cpu().numpy()
+    x = img.cpu().numpy()
+    threads = len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+
+    def cpu_step():
+        y = O.forward(x, minv, b, src, dst, mode="torch", zero_rule="thresh")
+        O.back(y.astype(np.float32), (n, n), minv, b, src, dst, mode="torch", zero_rule="thresh")
+
+    cpu_step()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        cpu_step()
+        times.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(times))
+    res = {
+        "metric": "2D fan-beam forward + back: rays/s (256x256, 360 views x 512 bins)",
+        "n_ray": n_ray, "steps": steps,
+        "gpu": {"ms_per_step": 1e3 * t_gpu, "rays_per_s": n_ray / t_gpu},
+        "cpu_baseline": {"ms_per_step": 1e3 * t_cpu, "rays_per_s": n_ray / t_cpu, "cores": threads, "kind": "port",
+                         "sample": f"full problem, median of 3 after a warm-up, {threads} threads of {cpu_model()}"},
+        "data": "synthetic (seeded ellipse phantom)",
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def batched_bench(a, dev) -> int:
     """64 independent 2D scans (sizes 128^2-512^2, 90-720 views, half fan beam, half
     StaticCT ring; scan i seeded by default_rng(100 + i)): forward + back of all
@@ -324,6 +385,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve", action="store_true",
                     help="served MAP stream instead of the projector pair: Poisson arrivals, continuous batching "
                          "vs the plain per-request loop (p50/p99 latency, throughput)")
+    ap.add_argument("--fan2d", action="store_true",
+                    help="the 2D fan-beam configuration (256^2, 360 views x 512 bins) with its CPU baseline")
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven 2D scans: one persistent launch (autotuned) vs 64 sequential launches")
     ap.add_argument("--map", action="store_true",
@@ -350,6 +413,8 @@ def main(argv=None) -> int:
         return map_bench(a, dev)
     if a.batched:
         return batched_bench(a, dev)
+    if a.fan2d:
+        return fan2d_bench(a, dev)
     if a.copy_calibration:
         x = torch.ones(1 << 28, device=dev)  # 1 GiB of float32
         y = torch.empty_like(x)
