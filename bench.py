@@ -281,7 +281,7 @@ def serve_bench(a, dev) -> int:
 
 
 def map_bench(a, dev) -> int:
-    """Config 3 at full size: MAP with the quadratic smoothness prior (weight
+    """Full-size MAP reconstruction: quadratic smoothness prior (weight
     1e-2, non-periodic) on a 512x512x256 volume at 0.75 mm, 600 views, 512x384
     detector at 1.125 mm, SID 1000 / SDD 1500 (117,964,800 rays).  Data = A
     x_true + sigma N(0,1) (default_rng(3)), sigma = 0.01 max(A x_true).  Times
