@@ -214,12 +214,13 @@ def batched_bench(a, dev) -> int:
         "classes": {c: sum(sc.geometry_class == c for sc in scans) for c in ("fan", "ring")},
         "sequential": {"ms": 1e3 * t_seq, "rays_per_s": n_rays / t_seq},
         "batched": {"ms": 1e3 * t_bat, "rays_per_s": n_rays / t_bat, "rays_per_item": best[0],
-                    "groups_per_cu": best[1], "rays_per_part": best[2]},
+                    "groups_per_cu": best[1], "rays_per_part": best[2], "tile": f"{best[3]}x64"},
         "speedup": t_seq / t_bat,
         "batched_atomic_back": {"ms": 1e3 * t_atomic, "rays_per_s": n_rays / t_atomic},
         "forward_bit_identical": same,
         "tuner_ms": {"forward": {f"{k[0]}x{k[1]}": v for k, v in sorted(timing["forward"].items())},
-                     "back": {str(k): v for k, v in sorted(timing["back"].items())}},
+                     "back": {f"{k[0]}x64 tiles, {k[1]} rays per part": v
+                              for k, v in sorted(timing["back"].items())}},
     }
     print(json.dumps(res), flush=True)
     return 0

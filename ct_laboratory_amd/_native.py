@@ -40,8 +40,8 @@ _SIGS = {
     "ctlab_zero_f64": [_P, _I64, _P],
     "ctlab_forward_project_2d_batch": [_P, _I64, _I64, _I64, _I64, _P, _P],
     "ctlab_back_project_2d_batch": [_P, _I64, _I64, _I64, _I64, _P, _INT, _P],
-    "ctlab_back_project_2d_batch_tiled": [_P, _I64, _I64, _I64, _I64, _P, _INT, _P],
-    "ctlab_back_project_2d_tile_units": [_I64, _I64, _I64, _I64],
+    "ctlab_back_project_2d_batch_tiled": [_P, _I64, _I64, _I64, _I64, _I64, _P, _INT, _P],
+    "ctlab_back_project_2d_tile_units": [_I64, _I64, _I64, _I64, _I64],
     "ctlab_forward_project_3d_otf": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I64, _I64, _INT, _P,
                                   _P],

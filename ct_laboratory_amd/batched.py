@@ -6,16 +6,16 @@ batch is one persistent launch:
 
 - forward (``ctlab_forward_project_2d_batch``): workgroups take work items of
   ``rays_per_item`` consecutive rays from a per-batch queue, one ray per lane;
-- back (``ctlab_back_project_2d_batch_tiled``): work units are (128x64 image
-  tile, ``rays_per_part`` consecutive rays); a unit sums into LDS and writes
-  its tile once, so no segment costs a global atomic.  ``tiled_back=False``
+- back (``ctlab_back_project_2d_batch_tiled``): work units are (``tile_rows``
+  x 64 image tile, ``rays_per_part`` consecutive rays); a unit sums into LDS
+  and writes its tile once, so no segment costs a global atomic.  ``tiled_back=False``
   selects the one-ray-per-lane atomic launch (``ctlab_back_project_2d_batch``)
   instead.
 
 ``autotune`` chooses ``rays_per_item`` and the number of persistent
-workgroups per CU for the forward, and ``rays_per_part`` for the back
-projection, by timing the batch's own projections; results are cached per
-geometry-class signature of the batch.
+workgroups per CU for the forward, and the tile size (``tile_rows``) and
+``rays_per_part`` for the back projection, by timing the batch's own
+projections; results are cached per geometry-class signature of the batch.
 
 Per scan the segments are those of the single-scan on-the-fly kernels
 (``_C.forward/back_project_2d_on_the_fly_cuda``), so the forward projections
@@ -41,7 +41,7 @@ SCAN_DTYPE = np.dtype([("image", "<u8"), ("sino", "<u8"), ("src", "<u8"), ("dst"
 assert SCAN_DTYPE.itemsize == 88
 
 N_CU = 256  # MI355X; the projector reads the device's own count
-# Tiled back projection: workgroups of 1024 lanes with 75 KB of LDS, two per CU.
+# Tiled back projection: workgroups of 1024 lanes with up to 75 KB of LDS, two per CU.
 TILED_GROUPS_PER_CU = 2
 
 
@@ -74,7 +74,7 @@ class BatchedProjector2D:
     _tuned: dict = {}
 
     def __init__(self, scans, device=None, rays_per_item=1024, groups_per_cu=4, tuner_file=None, rays_per_part=32768,
-                 tiled_back=True):
+                 tiled_back=True, tile_rows=128):
         self.scans = list(scans)
         self.device = torch.device("cuda") if device is None else torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:  # tensors always carry the index
@@ -88,13 +88,15 @@ class BatchedProjector2D:
         self.images = [torch.empty(sc.n_row, sc.n_col, device=self.device) for sc in self.scans]
         self.tuner_file = tuner_file
         self.rays_per_item, self.groups_per_cu = int(rays_per_item), int(groups_per_cu)
-        self.rays_per_part, self.tiled_back = int(rays_per_part), bool(tiled_back)
+        self.rays_per_part, self.tiled_back, self.tile_rows = int(rays_per_part), bool(tiled_back), int(tile_rows)
+        if self.tile_rows not in (64, 128):
+            raise ValueError("tile_rows must be 64 or 128")
         key = self.class_signature()
         cached = self._load_tuned().get(key)
         if cached:
             self.rays_per_item, self.groups_per_cu = cached[:2]
-            if len(cached) > 2:
-                self.rays_per_part = cached[2]
+            if len(cached) > 3:
+                self.rays_per_part, self.tile_rows = cached[2], cached[3]
         self._tables = {}
 
     # -- tuner cache ----------------------------------------------------------
@@ -111,9 +113,9 @@ class BatchedProjector2D:
         return BatchedProjector2D._tuned
 
     # -- descriptor table -------------------------------------------------------
-    def _build(self, images, sinos, rays_per_item, tiled=False):
+    def _build(self, images, sinos, rays_per_item, tiled=False, tile_rows=128):
         """Descriptor table and total work count: items of rays_per_item rays,
-        or (tiled) units of 128x64 tiles x rays_per_item rays."""
+        or (tiled) units of tile_rows x 64 tiles x rays_per_item rays."""
         lib = N.load() if tiled else None
         rec = np.zeros(len(self.scans), SCAN_DTYPE)
         begin = 0
@@ -128,22 +130,25 @@ class BatchedProjector2D:
             rec[k]["mode"], rec[k]["zero_rule"] = sc.mode, sc.zero_rule
             rec[k]["item_begin"] = begin
             if tiled:
-                begin += int(lib.ctlab_back_project_2d_tile_units(sc.n_row, sc.n_col, sc.n_ray, rays_per_item))
+                begin += int(lib.ctlab_back_project_2d_tile_units(sc.n_row, sc.n_col, sc.n_ray, rays_per_item,
+                                                                  tile_rows))
             else:
                 begin += -(-sc.n_ray // rays_per_item)
         table = torch.from_numpy(rec.view(np.uint8).copy()).to(self.device)
         return table, begin
 
-    def _launch(self, back, images, sinos, rays_per_item, groups_per_cu, accumulate=False, tiled=False):
+    def _launch(self, back, images, sinos, rays_per_item, groups_per_cu, accumulate=False, tiled=False,
+                tile_rows=128):
         """One persistent launch; for a tiled back projection rays_per_item is
         the rays per part."""
         for t in list(images) + list(sinos):
             if t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device:
                 raise RuntimeError("batched projector: images and sinograms must be contiguous float32 on the device")
         tiled = bool(back and tiled)
-        key = (tuple(t.data_ptr() for t in images), tuple(t.data_ptr() for t in sinos), rays_per_item, tiled)
+        key = (tuple(t.data_ptr() for t in images), tuple(t.data_ptr() for t in sinos), rays_per_item, tiled,
+               tile_rows)
         if key not in self._tables:
-            table, items = self._build(images, sinos, rays_per_item, tiled)
+            table, items = self._build(images, sinos, rays_per_item, tiled, tile_rows)
             # the pointed-to tensors stay referenced as long as the table
             self._tables[key] = (table, items, list(images), list(sinos))
             while len(self._tables) > 8:
@@ -153,7 +158,8 @@ class BatchedProjector2D:
         with torch.cuda.device(self.device):
             if tiled:
                 N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(self.scans), items, rays_per_item,
-                       TILED_GROUPS_PER_CU * self.n_cu, N.ptr(self.counter), int(bool(accumulate)), stream)
+                       tile_rows, TILED_GROUPS_PER_CU * self.n_cu, N.ptr(self.counter), int(bool(accumulate)),
+                       stream)
             elif back:
                 N.call("ctlab_back_project_2d_batch", N.ptr(table), len(self.scans), items, rays_per_item,
                        groups_per_cu * self.n_cu, N.ptr(self.counter), int(bool(accumulate)), stream)
@@ -176,7 +182,7 @@ class BatchedProjector2D:
             if tuple(s.shape) != (sc.n_ray,):
                 raise RuntimeError("batched projector: sinogram shape does not match its scan")
         if self.tiled_back:
-            self._launch(True, self.images, sinos, self.rays_per_part, 0, tiled=True)
+            self._launch(True, self.images, sinos, self.rays_per_part, 0, tiled=True, tile_rows=self.tile_rows)
         else:
             self._launch(True, self.images, sinos, self.rays_per_item, self.groups_per_cu)
         return self.images
@@ -192,27 +198,29 @@ class BatchedProjector2D:
         return ev[0].elapsed_time(ev[1]) / reps
 
     def autotune(self, images, rays_per_item=(256, 512, 1024, 2048, 4096), groups_per_cu=(1, 2, 4, 8, 16),
-                 rays_per_part=(8192, 16384, 32768, 65536, 131072), reps=3):
+                 rays_per_part=(8192, 16384, 32768, 65536, 131072), tile_rows=(64, 128), reps=3):
         """Times this batch's forward for every (rays_per_item, groups_per_cu),
-        and its back projection for every rays_per_part (tiled) or every
-        (rays_per_item, groups_per_cu) (atomic), and keeps the fastest.
-        Returns ((rays_per_item, groups_per_cu, rays_per_part), {"forward":
-        {config: ms}, "back": {config: ms}})."""
+        and its back projection for every (tile_rows, rays_per_part) (tiled) or
+        every (rays_per_item, groups_per_cu) (atomic), and keeps the fastest.
+        Returns ((rays_per_item, groups_per_cu, rays_per_part, tile_rows),
+        {"forward": {config: ms}, "back": {config: ms}})."""
         sinos = [torch.empty(sc.n_ray, device=self.device) for sc in self.scans]
         fwd = {(rpi, gpc): self._time(lambda: self._launch(False, images, sinos, rpi, gpc), reps)
                for rpi, gpc in itertools.product(rays_per_item, groups_per_cu)}
         if self.tiled_back:
-            back = {rpp: self._time(lambda: self._launch(True, self.images, sinos, rpp, 0, tiled=True), reps)
-                    for rpp in rays_per_part}
-            best_fwd, best_rpp = min(fwd, key=fwd.get), min(back, key=back.get)
+            back = {(tr, rpp): self._time(lambda: self._launch(True, self.images, sinos, rpp, 0, tiled=True,
+                                                               tile_rows=tr), reps)
+                    for tr, rpp in itertools.product(tile_rows, rays_per_part)}
+            best_fwd = min(fwd, key=fwd.get)
+            best_tr, best_rpp = min(back, key=back.get)
         else:
             back = {(rpi, gpc): self._time(lambda: self._launch(True, self.images, sinos, rpi, gpc), reps)
                     for rpi, gpc in itertools.product(rays_per_item, groups_per_cu)}
             # one (rays_per_item, groups_per_cu) serves both launches
             best_fwd = min(fwd, key=lambda k: fwd[k] + back[k])
-            best_rpp = self.rays_per_part
-        best = (best_fwd[0], best_fwd[1], best_rpp)
-        self.rays_per_item, self.groups_per_cu, self.rays_per_part = best
+            best_tr, best_rpp = self.tile_rows, self.rays_per_part
+        best = (best_fwd[0], best_fwd[1], best_rpp, best_tr)
+        self.rays_per_item, self.groups_per_cu, self.rays_per_part, self.tile_rows = best
         BatchedProjector2D._tuned[self.class_signature()] = list(best)
         if self.tuner_file:
             with open(self.tuner_file, "w") as f:

@@ -1,7 +1,7 @@
 // ctlab_tile2d.hip -- atomic-free 2D back projection over image tiles, for any
 // ray geometry (fan beam, static rings, parallel beams).
 //
-// A work unit is (tile, part): a 128x64 tile of one image and a contiguous
+// A work unit is (tile, part): a 128x64 (or 64x64) tile of one image and a contiguous
 // range of that scan's rays.  The unit's workgroup sums the range's
 // contributions to the tile in LDS as 64-bit fixed point (exact and
 // independent of the order of the rays, as in the 3D brick back projector)
@@ -21,12 +21,13 @@
 
 namespace ctlab {
 
-// Tile 128 x 64 pixels: 66.5 KB of LDS, two workgroups per CU.  Larger tiles
-// mean fewer (ray, tile) pairs to filter and set up: 64 x 64 measured 10.0 ms
-// on the 64-scan batch, 128 x 64 7.2 ms, 128 x 128 at one workgroup per CU 8.1.
-constexpr int kTileR = 128, kTileC = 64;       // tile rows x columns, pixels
+// Tiles are TR x 64 pixels, TR = 128 (66.5 KB of LDS) or 64 (33 KB), two
+// workgroups per CU either way.  Larger tiles mean fewer (ray, tile) pairs to
+// filter and set up: on the 64-scan batch 64 x 64 measured 10.0 ms, 128 x 64
+// 7.2 ms, and 128 x 128 at one workgroup per CU 8.1.  The single-scan launch
+// uses 128 x 64; the multi-scan launch takes the shape as an argument (tuned).
+constexpr int kTileC = 64;                     // tile columns, pixels
 constexpr int kTileStride = kTileC + 1;        // LDS row stride (64-bit cells), odd
-constexpr int kTileCells = kTileR * kTileStride;
 constexpr int kTileThreads = 1024;             // 16 waves; two workgroups per CU
 constexpr int kTilePerThread = 4;
 constexpr int kTileChunk = kTilePerThread * kTileThreads;  // rays filtered per chunk (< 2^16)
@@ -39,8 +40,9 @@ static_assert(sizeof(ctlab_scan2d) == 88, "ctlab_scan2d layout");
 // full-width rays).
 constexpr int kTileBuckets = 32;
 
+template <int TR>
 struct TileLds {
-    unsigned long long acc[kTileCells];
+    unsigned long long acc[TR * kTileStride];
     uint16_t queue[kTileChunk];
     float red[kTileThreads / 64];
     uint32_t count;
@@ -111,15 +113,15 @@ __device__ __forceinline__ float tile_block_max(float v, float* red) {
 // One work unit: rays [r0, r1) of a scan into tile `tile` of its image.
 // direct: this part is the tile's only one (plain stores; else float atomics
 // into an image that is zeroed or accumulated into).
-template <int MODE, bool DIAG>
+template <int MODE, bool DIAG, int TR>
 __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const float* __restrict__ src,
                           const float* __restrict__ dst, float* __restrict__ img, int tile, int tiles_c, int64_t r0,
-                          int64_t r1, bool direct, int accumulate, TileLds& L) {
+                          int64_t r1, bool direct, int accumulate, TileLds<TR>& L) {
     const int tid = threadIdx.x;
     Window<2> win;
-    win.lo[0] = (tile / tiles_c) * kTileR;
+    win.lo[0] = (tile / tiles_c) * TR;
     win.lo[1] = (tile % tiles_c) * kTileC;
-    win.ext[0] = min(kTileR, g.n[0] - win.lo[0]);
+    win.ext[0] = min(TR, g.n[0] - win.lo[0]);
     win.ext[1] = min(kTileC, g.n[1] - win.lo[1]);
     win.stride[0] = kTileStride;
     win.stride[1] = 1;
@@ -127,7 +129,7 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     // over the full tile (shifts, not divisions), skipping those past a ragged edge
     auto gidx = [&](int r, int c) -> int64_t { return (int64_t)(win.lo[0] + r) * g.n[1] + (win.lo[1] + c); };
 
-    for (int i = tid; i < kTileCells; i += kTileThreads) L.acc[i] = 0ull;
+    for (int i = tid; i < TR * kTileStride; i += kTileThreads) L.acc[i] = 0ull;
 
     // 1. bound on |value| over the part's rays (INF when one is INF or NaN)
     float vmax = 0.0f;
@@ -144,7 +146,7 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     const FixedScale fs = fixed_scale(2.0 * (double)vmax * pixel_diagonal(g), (double)(r1 - r0), g.finite);
 
     if (!fs.fixed && direct && !accumulate) {  // float atomics below need a zeroed tile
-        for (int p = tid; p < kTileR * kTileC; p += kTileThreads)
+        for (int p = tid; p < TR * kTileC; p += kTileThreads)
             if (p / kTileC < win.ext[0] && p % kTileC < win.ext[1]) img[gidx(p / kTileC, p % kTileC)] = 0.0f;
     }
     __syncthreads();
@@ -211,7 +213,7 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     // 4. write the tile (rows of the image, coalesced)
     if (!fs.fixed) return;
     const double inv = ldexp(1.0, -fs.F);
-    for (int p = tid; p < kTileR * kTileC; p += kTileThreads) {
+    for (int p = tid; p < TR * kTileC; p += kTileThreads) {
         const int r = p / kTileC, c = p % kTileC;
         if (r >= win.ext[0] || c >= win.ext[1]) continue;
         const float v = (float)((double)(long long)L.acc[r * kTileStride + c] * inv);
@@ -223,15 +225,17 @@ __device__ void tile_unit(const Grid<2>& g, const float* __restrict__ sn, const 
     }
 }
 
-template <int MODE>
+template <int MODE, int TR>
 __device__ __forceinline__ void tile_unit_any(const Grid<2>& g, const float* sn, const float* src, const float* dst,
                                               float* img, int tile, int tiles_c, int64_t r0, int64_t r1, bool direct,
-                                              int accumulate, TileLds& L) {
+                                              int accumulate, TileLds<TR>& L) {
     if (g.diag)
-        tile_unit<MODE, true>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+        tile_unit<MODE, true, TR>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
     else
-        tile_unit<MODE, false>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+        tile_unit<MODE, false, TR>(g, sn, src, dst, img, tile, tiles_c, r0, r1, direct, accumulate, L);
 }
+
+constexpr int kSingleScanTileRows = 128;
 
 // One scan, batch elements along y: unit x = part * n_tiles + tile (the tiles
 // of one ray range run together and share its rays in L2).
@@ -240,23 +244,25 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, const float* __restrict__ src,
     const float* __restrict__ dst, const float* __restrict__ minv, const float* __restrict__ b, int n_row, int n_col,
     int zero_rule, int tiles_c, int n_tiles, int64_t rays_per_part, int n_parts, int accumulate) {
-    __shared__ TileLds L;
+    __shared__ TileLds<kSingleScanTileRows> L;
     const int n[2] = {n_row, n_col};
     const Grid<2> g = load_grid<2>(n, minv, b, zero_rule);
     const int part = blockIdx.x / n_tiles, tile = blockIdx.x % n_tiles;
     const int64_t r0 = (int64_t)part * rays_per_part, r1 = min(r0 + rays_per_part, n_ray);
     const int64_t bi = blockIdx.y;
-    tile_unit_any<MODE>(g, sino + bi * n_ray, src, dst, out + bi * (int64_t)n_row * n_col, tile, tiles_c, r0, r1,
+    tile_unit_any<MODE, kSingleScanTileRows>(g, sino + bi * n_ray, src, dst, out + bi * (int64_t)n_row * n_col, tile,
+                                             tiles_c, r0, r1,
                         n_parts == 1, accumulate, L);
 }
 
 // Many scans, persistent: workgroups take units from a counter.  Scan k owns
 // units [item_begin_k, item_begin_k + tiles_k * parts_k), unit = part * tiles_k
-// + tile, parts_k = ceil(n_ray_k / rays_per_part).
+// + tile, parts_k = ceil(n_ray_k / rays_per_part), tiles of TR x 64 pixels.
+template <int TR>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2d_tiles(
     const ctlab_scan2d* __restrict__ scans, int n_scan, int64_t total_units, int64_t rays_per_part, int accumulate,
     unsigned long long* __restrict__ counter) {
-    __shared__ TileLds L;
+    __shared__ TileLds<TR> L;
     __shared__ int64_t s_unit;
     while (true) {
         if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd(counter, 1ull);
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         const int n[2] = {sc.n_row, sc.n_col};
         const Grid<2> g = load_grid<2>(n, sc.minv, sc.b, sc.zero_rule);
         const int tiles_c = (sc.n_col + kTileC - 1) / kTileC;
-        const int n_tiles = ((sc.n_row + kTileR - 1) / kTileR) * tiles_c;
+        const int n_tiles = ((sc.n_row + TR - 1) / TR) * tiles_c;
         const int64_t local = unit - sc.item_begin;
         const int64_t part = local / n_tiles;
         const int tile = (int)(local % n_tiles);
@@ -285,9 +291,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         const float* d = reinterpret_cast<const float*>(sc.dst);
         float* img = reinterpret_cast<float*>(sc.image);
         if (sc.mode == MODE_OTF)
-            tile_unit_any<MODE_OTF>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+            tile_unit_any<MODE_OTF, TR>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
         else
-            tile_unit_any<MODE_TORCH>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
+            tile_unit_any<MODE_TORCH, TR>(g, sn, s, d, img, tile, tiles_c, r0, r1, direct, accumulate, L);
         __syncthreads();  // the tile's LDS is cleared by the next unit
     }
 }
@@ -299,7 +305,7 @@ int back_project_2d_tiles(const char* fn, const float* sino, int64_t batch, int6
                           const float* dst, const float* minv, const float* b, const int64_t* n, int mode,
                           int zero_rule, int accumulate, float* out, hipStream_t stream) {
     const int tiles_c = (int)((n[1] + kTileC - 1) / kTileC);
-    const int64_t n_tiles = ((n[0] + kTileR - 1) / kTileR) * tiles_c;
+    const int64_t n_tiles = ((n[0] + kSingleScanTileRows - 1) / kSingleScanTileRows) * tiles_c;
     CTLAB_REQUIRE(n_tiles <= (1LL << 24), "%s: image too large for the tile grid", fn);
     CTLAB_REQUIRE(n_ray >= 1, "%s: no rays", fn);
     int64_t parts = (1024 + n_tiles * batch - 1) / (n_tiles * batch);  // about 1024 units
@@ -330,25 +336,30 @@ int back_project_2d_tiles(const char* fn, const float* sino, int64_t batch, int6
 using namespace ctlab;
 
 extern "C" int64_t ctlab_back_project_2d_tile_units(int64_t n_row, int64_t n_col, int64_t n_ray,
-                                                     int64_t rays_per_part) {
-    if (n_row < 1 || n_col < 1 || n_ray < 0 || rays_per_part < 1) return -1;
-    const int64_t tiles = ((n_row + kTileR - 1) / kTileR) * ((n_col + kTileC - 1) / kTileC);
+                                                     int64_t rays_per_part, int64_t tile_rows) {
+    if (n_row < 1 || n_col < 1 || n_ray < 0 || rays_per_part < 1 || (tile_rows != 64 && tile_rows != 128)) return -1;
+    const int64_t tiles = ((n_row + tile_rows - 1) / tile_rows) * ((n_col + kTileC - 1) / kTileC);
     return tiles * ((n_ray + rays_per_part - 1) / rays_per_part);
 }
 
 extern "C" int ctlab_back_project_2d_batch_tiled(const ctlab_scan2d* scans, int64_t n_scan, int64_t total_units,
-                                                 int64_t rays_per_part, int64_t n_groups, void* counter,
-                                                 int accumulate, ctlab_stream_t stream) {
+                                                 int64_t rays_per_part, int64_t tile_rows, int64_t n_groups,
+                                                 void* counter, int accumulate, ctlab_stream_t stream) {
     const char* fn = "back_project_2d_batch_tiled";
     CTLAB_REQUIRE(n_scan >= 0 && n_scan <= 0x7fffffffLL, "%s: n_scan out of range", fn);
     CTLAB_REQUIRE(total_units >= 0 && rays_per_part >= 1 && rays_per_part <= (1LL << 40), "%s: bad work units", fn);
+    CTLAB_REQUIRE(tile_rows == 64 || tile_rows == 128, "%s: tile_rows %lld (64 or 128)", fn, (long long)tile_rows);
     CTLAB_REQUIRE(n_groups >= 1 && n_groups <= 65536, "%s: n_groups %lld out of range", fn, (long long)n_groups);
     if (n_scan == 0) return CTLAB_OK;
     CTLAB_REQUIRE(scans && counter, "%s: null pointer", fn);
     auto* ctr = reinterpret_cast<unsigned long long*>(counter);
     int st = batch2d_clear(fn, scans, n_scan, !accumulate, ctr, as_stream(stream));
     if (st || total_units == 0) return st;
-    hipLaunchKernelGGL(k_batch2d_tiles, dim3((unsigned)n_groups), dim3(kTileThreads), 0, as_stream(stream), scans,
-                       (int)n_scan, total_units, rays_per_part, accumulate, ctr);
+    if (tile_rows == 128)
+        hipLaunchKernelGGL(k_batch2d_tiles<128>, dim3((unsigned)n_groups), dim3(kTileThreads), 0, as_stream(stream),
+                           scans, (int)n_scan, total_units, rays_per_part, accumulate, ctr);
+    else
+        hipLaunchKernelGGL(k_batch2d_tiles<64>, dim3((unsigned)n_groups), dim3(kTileThreads), 0, as_stream(stream),
+                           scans, (int)n_scan, total_units, rays_per_part, accumulate, ctr);
     return check_launch(fn);
 }

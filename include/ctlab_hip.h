@@ -264,22 +264,24 @@ int ctlab_back_project_2d_batch(const ctlab_scan2d* scans, int64_t n_scan, int64
 
 /* Back projections of all scans without global atomics per segment (no
  * counterpart in the reference; same results as ctlab_back_project_2d_batch
- * within float summation order).  Work units are (128x64 image tile, range of
- * rays_per_part consecutive rays): in this call item_begin is the scan's first
- * UNIT, and scan k owns ceil(n_row/128) * ceil(n_col/64) * ceil(n_ray/rays_per_part)
- * units (ctlab_back_project_2d_tile_units); total_units is their sum.  A unit sums its rays' contributions to its
+ * within float summation order).  Work units are (tile_rows x 64 image tile,
+ * tile_rows = 128 or 64, range of rays_per_part consecutive rays): in this call
+ * item_begin is the scan's first UNIT, and scan k owns ceil(n_row/tile_rows) *
+ * ceil(n_col/64) * ceil(n_ray/rays_per_part) units
+ * (ctlab_back_project_2d_tile_units); total_units is their sum.  A unit sums its rays' contributions to its
  * tile in LDS (64-bit fixed point, order independent) and writes the tile
  * once: with plain stores when the scan has one part, else with one float
  * atomic add per pixel.  Images are zeroed first unless accumulate != 0.
  * Units past a scan's last one (item_begin of the next scan, or total_units,
  * set too high) are skipped. */
 int ctlab_back_project_2d_batch_tiled(const ctlab_scan2d* scans, int64_t n_scan, int64_t total_units,
-                                      int64_t rays_per_part, int64_t n_groups, void* counter, int accumulate,
-                                      ctlab_stream_t stream);
+                                      int64_t rays_per_part, int64_t tile_rows, int64_t n_groups, void* counter,
+                                      int accumulate, ctlab_stream_t stream);
 
 /* Number of work units of one scan in ctlab_back_project_2d_batch_tiled
- * (-1 for invalid sizes). */
-int64_t ctlab_back_project_2d_tile_units(int64_t n_row, int64_t n_col, int64_t n_ray, int64_t rays_per_part);
+ * (-1 for invalid sizes or tile_rows other than 64 and 128). */
+int64_t ctlab_back_project_2d_tile_units(int64_t n_row, int64_t n_col, int64_t n_ray, int64_t rays_per_part,
+                                         int64_t tile_rows);
 
 /* ---------------------------------------------------------------------- */
 /* MAP iteration with a Gaussian likelihood and the quadratic smoothness   */

@@ -46,12 +46,12 @@ def _single(sc, img, dev):
     return _C.forward_project_2d_on_the_fly_cuda(img, sc.src, sc.dst, minv, b, mode=sc.mode, zero_rule=sc.zero_rule)[0]
 
 
-@pytest.mark.parametrize("rpi,gpc,rpp,tiled", [(256, 1, 64, True), (512, 4, 4096, True), (4096, 16, 1 << 20, True),
-                                              (512, 4, 0, False)])
-def test_batch_matches_single_scan_kernels(dev, rpi, gpc, rpp, tiled):
+@pytest.mark.parametrize("rpi,gpc,rpp,tiled,tr", [(256, 1, 64, True, 128), (512, 4, 4096, True, 64),
+                                                 (4096, 16, 1 << 20, True, 128), (512, 4, 0, False, 128)])
+def test_batch_matches_single_scan_kernels(dev, rpi, gpc, rpp, tiled, tr):
     scans, images = _scans(dev)
     bp = BatchedProjector2D(scans, dev, rays_per_item=rpi, groups_per_cu=gpc, rays_per_part=max(rpp, 1),
-                            tiled_back=tiled)
+                            tiled_back=tiled, tile_rows=tr)
     sinos = bp.forward(images)
     ys = []
     for sc, img, s in zip(scans, images, sinos):
@@ -79,9 +79,9 @@ def test_autotune_and_cache(dev, tmp_path):
     best, timing = bp.autotune(images, rays_per_item=(256, 1024), groups_per_cu=(1, 4), rays_per_part=(512, 8192),
                                reps=1)
     assert best[:2] in timing["forward"] and len(timing["forward"]) == 4 and f.exists()
-    assert best[2] in timing["back"] and len(timing["back"]) == 2
+    assert (best[3], best[2]) in timing["back"] and len(timing["back"]) == 4
     bp2 = BatchedProjector2D(scans, dev, tuner_file=str(f))
-    assert (bp2.rays_per_item, bp2.groups_per_cu, bp2.rays_per_part) == best
+    assert (bp2.rays_per_item, bp2.groups_per_cu, bp2.rays_per_part, bp2.tile_rows) == best
     sinos = bp2.forward(images)
     assert all(torch.equal(s, _single(sc, im, dev)) for s, sc, im in zip(sinos, scans, images))
 
@@ -95,12 +95,12 @@ def test_tiled_batch_accumulate_and_spare_units(dev):
     ys = [torch.from_numpy(np.random.default_rng(k).uniform(0, 1, sc.n_ray).astype(np.float32)).to(dev)
           for k, sc in enumerate(scans)]
     ref = [b.clone() for b in bp.back(ys)]
-    table, units = bp._build(bp.images, ys, 512, tiled=True)
+    table, units = bp._build(bp.images, ys, 512, tiled=True, tile_rows=128)
     base = [torch.full_like(im, 0.25) for im in bp.images]
     for k in range(len(scans)):
         bp.images[k].copy_(base[k])
     stream = N.stream_of(bp.counter)
-    N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(scans), units + 37, 512, 2 * bp.n_cu,
+    N.call("ctlab_back_project_2d_batch_tiled", N.ptr(table), len(scans), units + 37, 512, 128, 2 * bp.n_cu,
            N.ptr(bp.counter), 1, stream)
     torch.cuda.synchronize()
     for k in range(len(scans)):
