@@ -51,13 +51,31 @@ class CTProjector2DModule(torch.nn.Module):
         self.register_buffer("tvals", tvals)
         self._own_table = None if tvals is None else (weakref.ref(tvals), tvals._version)
 
+    def _own_untouched(self) -> bool:
+        t = self.tvals
+        return t is not None and self._own_table is not None and self._own_table[0]() is t and \
+            t._version == self._own_table[1]
+
+    def __getstate__(self):
+        # a weak reference does not pickle (torch.save, copy.deepcopy): keep whether the
+        # table is still the module's own, untouched one, and point at the copy on load
+        state = self.__dict__.copy()
+        state["_own_table"] = self._own_untouched()
+        return state
+
+    def __setstate__(self, state):
+        own = state.pop("_own_table", False)
+        self.__dict__.update(state)
+        t = self._buffers.get("tvals")
+        self._own_table = (weakref.ref(t), t._version) if (own is True and t is not None) else None
+
     def _rule(self):
         """(tvals, semantics, zero_rule) to project with: no table and the
         table's rule while the table is the module's own, untouched one."""
         t = self.tvals
         if t is None:
             return None, self.semantics, None
-        if self._own_table is not None and self._own_table[0]() is t and t._version == self._own_table[1]:
+        if self._own_untouched():
             return None, "torch", N.ZERO_THRESH
         return t, self.semantics, None
 

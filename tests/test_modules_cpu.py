@@ -1,0 +1,51 @@
+"""CPU tests of the 2D module's own-table bookkeeping (no kernels run): the
+module projects on the fly only while its table is the one it computed and
+nothing has written to it, and that state survives deepcopy and pickling."""
+import copy
+import io
+import weakref
+
+import torch
+
+from ct_laboratory_amd import _native as N
+from ct_laboratory_amd.tomography.ct_projector_2d_module import CTProjector2DModule
+
+
+def _module_with_own_table():
+    m = CTProjector2DModule(8, 8, torch.eye(2), torch.zeros(2), torch.zeros(3, 2), torch.ones(3, 2),
+                            precomputed_intersections=False)
+    t = torch.zeros(3, 18)  # stands in for the table the constructor computes on the GPU
+    m.register_buffer("tvals", t)
+    m._own_table = (weakref.ref(t), t._version)
+    return m
+
+
+def test_own_table_rule_and_edits():
+    m = _module_with_own_table()
+    assert m._rule() == (None, "torch", N.ZERO_THRESH)
+    with torch.no_grad():
+        m.tvals[0, 0] = 0.5  # written in place: consumed as given from now on
+    assert m._rule()[0] is m.tvals
+    m2 = _module_with_own_table()
+    m2.tvals = torch.zeros(3, 18)  # replaced: consumed as given
+    assert m2._rule()[0] is m2.tvals
+
+
+def test_own_table_survives_copies():
+    m = _module_with_own_table()
+    c = copy.deepcopy(m)
+    assert c._rule()[0] is None and c.tvals is not m.tvals
+    buf = io.BytesIO()
+    torch.save(m, buf)
+    buf.seek(0)
+    r = torch.load(buf, weights_only=False)  # a file this test wrote
+    assert r._rule()[0] is None
+    with torch.no_grad():
+        m.tvals.add_(1.0)
+    assert copy.deepcopy(m)._rule()[0] is not None
+
+
+def test_no_table_uses_module_semantics():
+    m = CTProjector2DModule(8, 8, torch.eye(2), torch.zeros(2), torch.zeros(3, 2), torch.ones(3, 2),
+                            precomputed_intersections=False, semantics="otf")
+    assert m._rule() == (None, "otf", None)
