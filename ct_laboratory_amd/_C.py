@@ -12,12 +12,28 @@ Differences from the reference, all deliberate:
 
 Outputs are allocated here with torch (as the reference's host wrappers do)
 and carry a leading batch dimension.
+
+The 3D on-the-fly functions take rays without a detector layout, as the
+reference's call sites pass them.  When the rays are a stack of flat-panel
+views ([view][rows][cols], one source per view: the cone-beam and helical
+builders' order), the layout is recognised once per ray set (cached, keyed by
+the tensors' identity and version counters) and used: the forward gets the
+detector-tiled schedule (same results) and the back projection runs on LDS
+bricks instead of one global atomic per segment (same segments, fixed-point
+sums).  Recognising a layout reads the rays and synchronises once; a brick
+plan is cached per (ray set, grid, M_inv and b values).  ``AUTO_LAYOUT =
+False`` (or CTLAB_AUTO_LAYOUT=0 in the environment) turns it off.
 """
 from __future__ import annotations
+
+import os
+from collections import OrderedDict
 
 import torch
 
 from . import _native as N
+
+AUTO_LAYOUT = os.environ.get("CTLAB_AUTO_LAYOUT", "1") != "0"
 
 
 def _require(cond: bool, msg: str) -> None:
@@ -90,6 +106,8 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
     n_ray = _rays(name, src, dst, 3)
     _affine(name, M_inv, b, 3)
+    if det_shape is None:
+        det_shape = _layout_of(src, dst)
     out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
@@ -100,12 +118,21 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
 
 def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF,
                                     det_shape=None, out=None):
-    """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079)."""
+    """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079).
+
+    Without ``det_shape``, rays recognised as a stack of flat-panel views are
+    back-projected on LDS bricks (same segments); ``det_shape=(rows, cols)``
+    declares the layout and keeps the one-ray-per-lane kernel, scheduled by it."""
     name = "back_project_3d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     _affine(name, M_inv, b, 3)
+    if det_shape is None and batch <= 65535:
+        layout = _layout_of(src, dst)
+        plan = _plan_of(src, dst, layout, M_inv, b, (int(n_x), int(n_y), int(n_z))) if layout else None
+        if plan is not None:
+            return back_project_3d_brick(sino, src, dst, layout, plan, M_inv, b, n_x, n_y, n_z, mode=mode, out=out)
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
@@ -431,6 +458,89 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
                N.ptr(plan), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
                N.ptr(out), N.ptr(workspace), N.stream_of(sino))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Detector layout of 3D rays passed without one
+# ---------------------------------------------------------------------------
+
+_CACHE_SIZE = 4
+_MIN_VIEW_RAYS = 64
+_MIN_VALID_VIEWS = 0.9
+_LAYOUTS: OrderedDict = OrderedDict()  # ray identity -> (rows, cols) or None
+_PLANS: OrderedDict = OrderedDict()    # (ray identity, layout, grid, M_inv and b values) -> plan or None
+
+
+def _remember(cache: OrderedDict, key, value):
+    cache[key] = value
+    cache.move_to_end(key)
+    while len(cache) > _CACHE_SIZE:
+        cache.popitem(last=False)
+    return value
+
+
+def _ray_identity(src, dst):
+    return (str(src.device), src.data_ptr(), dst.data_ptr(), int(src.shape[0]), src._version, dst._version)
+
+
+def detect_layout(src, dst):
+    """(rows, cols) when rays [n_ray, 3] are ordered [view][rows][cols] with one
+    source per view and a detector row pitch that changes only between rows,
+    else None.  Reads the rays (one synchronisation)."""
+    n = int(src.shape[0])
+    if n < _MIN_VIEW_RAYS:
+        return None
+    probe = min(n, 1 << 22)
+    change = torch.nonzero((src[1:probe] != src[:1]).any(dim=1))
+    if change.numel() == 0:
+        if probe < n:
+            return None
+        per_view = n
+    else:
+        per_view = int(change[0]) + 1
+    if per_view < _MIN_VIEW_RAYS or n % per_view:
+        return None
+    s3 = src.view(n // per_view, per_view, 3)
+    if not bool((s3 == s3[:, :1]).all()):
+        return None
+    d = dst[:per_view].double()
+    steps = d[1:] - d[:-1]
+    dev = (steps - steps[:1]).abs().amax(dim=1)
+    breaks = torch.nonzero(dev > 1e-3 * float(steps[0].norm()) + 1e-12)
+    cols = int(breaks[0]) + 1 if breaks.numel() else per_view
+    if cols < 2 or per_view % cols:
+        return None
+    return (per_view // cols, cols)
+
+
+def _capturing() -> bool:
+    return torch.cuda.is_current_stream_capturing()
+
+
+def _layout_of(src, dst):
+    """Cached layout of a ray set; none while a graph is being captured (recognition
+    would synchronise), so captured work never depends on the cache."""
+    if not AUTO_LAYOUT or _capturing():
+        return None
+    key = _ray_identity(src, dst)
+    if key in _LAYOUTS:
+        _LAYOUTS.move_to_end(key)
+        return _LAYOUTS[key]
+    return _remember(_LAYOUTS, key, detect_layout(src, dst))
+
+
+def _plan_of(src, dst, layout, M_inv, b, shape):
+    """Brick plan for the recognised layout, or None when fewer than 90% of the
+    views fit the flat-panel model (the brick would then read whole detectors)."""
+    key = (_ray_identity(src, dst), layout, shape, tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()))
+    if key in _PLANS:
+        _PLANS.move_to_end(key)
+        return _PLANS[key]
+    frames = cone_frames(src, dst, layout, M_inv, b, shape)
+    plan = None
+    if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
+        plan = brick_plan(frames, layout, shape)
+    return _remember(_PLANS, key, plan)
 
 
 REFERENCE_NAMES = (

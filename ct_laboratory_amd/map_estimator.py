@@ -140,6 +140,12 @@ class ProjectorOps:
             opts = proj._options() if self.table is None else None
             self.det_shape = None if opts is None else opts.det_shape
             self.plan = None if opts is None else opts.plan
+            if self.table is None and self.plan is None:
+                # rays given without a layout: recognise it now, outside any graph capture
+                layout = _C._layout_of(self.src, self.dst)
+                plan = _C._plan_of(self.src, self.dst, layout, self.minv, self.b, tuple(self.shape)) if layout else None
+                if plan is not None:
+                    self.det_shape, self.plan = layout, plan
             self.ws = None
             if self.plan is not None:
                 n_view = self.n_ray // (self.det_shape[0] * self.det_shape[1])

@@ -49,3 +49,24 @@ def test_no_table_uses_module_semantics():
     m = CTProjector2DModule(8, 8, torch.eye(2), torch.zeros(2), torch.zeros(3, 2), torch.ones(3, 2),
                             precomputed_intersections=False, semantics="otf")
     assert m._rule() == (None, "otf", None)
+
+
+def test_detect_layout_of_ray_stacks():
+    """The _C shim recognises [view][rows][cols] stacks of flat-panel views (one
+    source per view) and nothing else."""
+    import numpy as np
+
+    from ct_laboratory_amd import _C
+    from ct_laboratory_amd.tomography import build_conebeam_3d_geometry
+
+    src, dst = build_conebeam_3d_geometry(5, 24, 10, 1000.0, 1500.0, det_spacing=2.0)  # 5 views, 10 x 24
+    assert _C.detect_layout(src.contiguous(), dst.contiguous()) == (10, 24)
+    one_row = build_conebeam_3d_geometry(3, 80, 1, 1000.0, 1500.0, det_spacing=2.0)
+    assert _C.detect_layout(*(t.contiguous() for t in one_row)) == (1, 80)
+    rng = np.random.default_rng(0)
+    r = torch.from_numpy(rng.uniform(-50, 50, (240, 3)).astype(np.float32))
+    assert _C.detect_layout(r, -r) is None  # a different source per ray
+    s2 = src.clone()
+    s2[7] += 1.0  # one view whose source moves within the view
+    assert _C.detect_layout(s2, dst.contiguous()) is None
+    assert _C.detect_layout(src[:40].contiguous(), dst[:40].contiguous()) is None  # too few rays per view
