@@ -21,12 +21,20 @@ the tensors' identity and version counters) and used: the forward gets the
 detector-tiled schedule (same results) and the back projection runs on LDS
 bricks instead of one global atomic per segment (same segments, fixed-point
 sums).  Recognising a layout reads the rays and synchronises once; a brick
-plan is cached per (ray set, grid, M_inv and b values).  ``AUTO_LAYOUT =
-False`` (or CTLAB_AUTO_LAYOUT=0 in the environment) turns it off.
+plan is cached per (ray set, grid, M_inv and b values).
+
+Likewise a 2D crossing table that compute_intersections_2d returned, passed
+back untouched with the same rays, grid and map, is projected on the fly with
+its own rule (the same segments in the same order: a bit-identical forward, and
+a back projection over LDS tiles instead of per-segment atomics).
+
+``AUTO_LAYOUT = False`` (or CTLAB_AUTO_LAYOUT=0 in the environment) turns both
+off.
 """
 from __future__ import annotations
 
 import os
+import weakref
 from collections import OrderedDict
 
 import torch
@@ -226,6 +234,7 @@ def compute_intersections_2d(n_row, n_col, src, dst, M_inv, b, *, zero_rule: int
     with torch.cuda.device(src.device):
         N.call("ctlab_compute_intersections_2d", int(n_row), int(n_col), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
                N.ptr(b), zero_rule, K, N.ptr(out), N.stream_of(src))
+    _register_table2d(out, n_row, n_col, src, dst, M_inv, b, zero_rule, K)
     return out
 
 
@@ -275,6 +284,10 @@ def forward_project_2d_cuda(img, tvals, src, dst, M_inv, b, *, out=None):
     n_ray = _rays(name, src, dst, 2)
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 2)
+    zr = _table2d_rule(tvals, src, dst, M_inv, b, nr, nc)
+    if zr is not None:  # a table this module computed, untouched: its segments on the fly
+        return forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, mode=N.MODE_TORCH, zero_rule=zr,
+                                                  out=out)
     out = _out(name, out, (batch, n_ray), img.device)
     with torch.cuda.device(img.device):
         N.call("ctlab_forward_project_2d_tvals", N.ptr(img), batch, nr, nc, N.ptr(tvals), K, N.ptr(src), N.ptr(dst),
@@ -290,6 +303,10 @@ def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col, *, out=N
     _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 2)
+    zr = _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col)
+    if zr is not None:  # a table this module computed, untouched: its segments on the fly, LDS tiles
+        return back_project_2d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_row, n_col, mode=N.MODE_TORCH,
+                                               zero_rule=zr, out=out)
     out = _out(name, out, (batch, n_row, n_col), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_2d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
@@ -541,6 +558,46 @@ def _plan_of(src, dst, layout, M_inv, b, shape):
     if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
         plan = brick_plan(frames, layout, shape)
     return _remember(_PLANS, key, plan)
+
+
+# ---------------------------------------------------------------------------
+# 2D crossing tables computed here
+# ---------------------------------------------------------------------------
+
+# id(table) -> (weak reference, version, (n_row, n_col, src ref, src version, dst ref, dst version,
+# M_inv values, b values, zero rule)) of full tables compute_intersections_2d returned
+_TABLES2D: dict = {}
+
+
+def _register_table2d(table, n_row, n_col, src, dst, M_inv, b, zero_rule, K):
+    if K != int(n_row) + int(n_col) + 2 or _capturing():
+        return
+    if len(_TABLES2D) > 64:
+        for k in [k for k, e in _TABLES2D.items() if e[0]() is None]:
+            del _TABLES2D[k]
+    _TABLES2D[id(table)] = (weakref.ref(table), table._version,
+                            (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
+                             tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()), int(zero_rule)))
+
+
+def _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col):
+    """The zero rule of a full table that compute_intersections_2d returned for
+    these very rays, grid and map and that nothing has written to since: its
+    segments are exactly those of the on-the-fly torch rule with that zero rule,
+    in the same order.  None otherwise (the table is consumed as given)."""
+    if not AUTO_LAYOUT or _capturing():
+        return None
+    e = _TABLES2D.get(id(tvals))
+    if e is None or e[0]() is not tvals or tvals._version != e[1]:
+        return None
+    nr, nc, sref, sver, dref, dver, mv, bv, zr = e[2]
+    if (nr, nc) != (int(n_row), int(n_col)) or sref() is not src or dref() is not dst:
+        return None
+    if src._version != sver or dst._version != dver:
+        return None
+    if tuple(M_inv.flatten().tolist()) != mv or tuple(b.flatten().tolist()) != bv:
+        return None
+    return zr
 
 
 REFERENCE_NAMES = (

@@ -57,11 +57,18 @@ def test_table_consumer_equals_on_the_fly(dev, nd):
         b_tab = _C.back_project_3d_cuda(sino, tab, ts, td, tm, tb, *shape)
         b_otf = _C.back_project_3d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH)
     else:
-        tab = _C.compute_intersections_2d(*shape, ts, td, tm, tb, zero_rule=zr)
+        own = _C.compute_intersections_2d(*shape, ts, td, tm, tb, zero_rule=zr)
+        tab = own.clone()  # a copy is consumed as given: the table kernels run
         f_tab = _C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb)
         f_otf = _C.forward_project_2d_on_the_fly_cuda(vol, ts, td, tm, tb, mode=N.MODE_TORCH, zero_rule=zr)
         b_tab = _C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape)
         b_otf = _C.back_project_2d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH, zero_rule=zr)
+        # the table as returned runs on the fly with its own rule
+        assert torch.equal(_C.forward_project_2d_cuda(vol, own, ts, td, tm, tb), f_otf)
+        assert torch.equal(_C.back_project_2d_cuda(sino, own, ts, td, tm, tb, *shape), b_otf)
+        with torch.no_grad():
+            own[0, 0] = float("inf")  # written: consumed as given again
+        assert _C._table2d_rule(own, ts, td, tm, tb, *shape) is None
     assert torch.equal(f_tab, f_otf)
     assert rel_err(b_tab.cpu().numpy(), b_otf.cpu().numpy()) < 1e-6
 
