@@ -34,6 +34,7 @@ off.
 from __future__ import annotations
 
 import os
+import threading
 import weakref
 from collections import OrderedDict
 
@@ -486,6 +487,7 @@ _MIN_VIEW_RAYS = 64
 _MIN_VALID_VIEWS = 0.9
 _LAYOUTS: OrderedDict = OrderedDict()  # ray identity -> (rows, cols) or None
 _PLANS: OrderedDict = OrderedDict()    # (ray identity, layout, grid, M_inv and b values) -> plan or None
+_CACHE_LOCK = threading.RLock()        # the caches are shared by the host threads of a process
 
 
 def _remember(cache: OrderedDict, key, value):
@@ -540,24 +542,26 @@ def _layout_of(src, dst):
     if not AUTO_LAYOUT or _capturing():
         return None
     key = _ray_identity(src, dst)
-    if key in _LAYOUTS:
-        _LAYOUTS.move_to_end(key)
-        return _LAYOUTS[key]
-    return _remember(_LAYOUTS, key, detect_layout(src, dst))
+    with _CACHE_LOCK:
+        if key in _LAYOUTS:
+            _LAYOUTS.move_to_end(key)
+            return _LAYOUTS[key]
+        return _remember(_LAYOUTS, key, detect_layout(src, dst))
 
 
 def _plan_of(src, dst, layout, M_inv, b, shape):
     """Brick plan for the recognised layout, or None when fewer than 90% of the
     views fit the flat-panel model (the brick would then read whole detectors)."""
     key = (_ray_identity(src, dst), layout, shape, tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()))
-    if key in _PLANS:
-        _PLANS.move_to_end(key)
-        return _PLANS[key]
-    frames = cone_frames(src, dst, layout, M_inv, b, shape)
-    plan = None
-    if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
-        plan = brick_plan(frames, layout, shape)
-    return _remember(_PLANS, key, plan)
+    with _CACHE_LOCK:
+        if key in _PLANS:
+            _PLANS.move_to_end(key)
+            return _PLANS[key]
+        frames = cone_frames(src, dst, layout, M_inv, b, shape)
+        plan = None
+        if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
+            plan = brick_plan(frames, layout, shape)
+        return _remember(_PLANS, key, plan)
 
 
 # ---------------------------------------------------------------------------
@@ -572,12 +576,14 @@ _TABLES2D: dict = {}
 def _register_table2d(table, n_row, n_col, src, dst, M_inv, b, zero_rule, K):
     if K != int(n_row) + int(n_col) + 2 or _capturing():
         return
-    if len(_TABLES2D) > 64:
-        for k in [k for k, e in _TABLES2D.items() if e[0]() is None]:
-            del _TABLES2D[k]
-    _TABLES2D[id(table)] = (weakref.ref(table), table._version,
-                            (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
-                             tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()), int(zero_rule)))
+    record = (weakref.ref(table), table._version,
+              (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
+               tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()), int(zero_rule)))
+    with _CACHE_LOCK:
+        if len(_TABLES2D) > 64:
+            for k in [k for k, e in _TABLES2D.items() if e[0]() is None]:
+                del _TABLES2D[k]
+        _TABLES2D[id(table)] = record
 
 
 def _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col):
@@ -587,7 +593,8 @@ def _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col):
     in the same order.  None otherwise (the table is consumed as given)."""
     if not AUTO_LAYOUT or _capturing():
         return None
-    e = _TABLES2D.get(id(tvals))
+    with _CACHE_LOCK:
+        e = _TABLES2D.get(id(tvals))
     if e is None or e[0]() is not tvals or tvals._version != e[1]:
         return None
     nr, nc, sref, sver, dref, dver, mv, bv, zr = e[2]
