@@ -153,6 +153,46 @@ def fan2d_bench(a, dev) -> int:
     return 0
 
 
+def refcalls_bench(a, dev) -> int:
+    """The 256^3 cone-beam workload through the _C functions called as the
+    reference's ct_projector_3d_cuda.py calls them (no detector layout, otf
+    rule): with the shim's layout recognition (tiled forward, brick back
+    projection) and without it (one ray per lane in storage order, global
+    atomics).  Prints one JSON line."""
+    import torch
+
+    from ct_laboratory_amd import _C
+
+    _, _, _, _, _, ts, td, tv, tm, tb = build_inputs(dev)
+    ts, td = ts.contiguous(), td.contiguous()
+    n = CFG["n"]
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    res = {"metric": "256^3 cone-beam through the reference-style _C calls: ms per call",
+           "workload": "720 views, 384x384 detector, otf rule, no det_shape passed"}
+    y = _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb)
+    for auto in (True, False):
+        _C.AUTO_LAYOUT = auto
+        try:
+            res["recognised" if auto else "plain"] = {
+                "forward_ms": timed(lambda: _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb), 3),
+                "back_ms": timed(lambda: _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, n, n, n), 3 if auto else 1),
+            }
+        finally:
+            _C.AUTO_LAYOUT = True
+    res["layout"] = _C._layout_of(ts, td)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def batched_bench(a, dev) -> int:
     """64 independent 2D scans (sizes 128^2-512^2, 90-720 views, half fan beam, half
     StaticCT ring; scan i seeded by default_rng(100 + i)): forward + back of all
@@ -388,6 +428,8 @@ def main(argv=None) -> int:
                          "vs the plain per-request loop (p50/p99 latency, throughput)")
     ap.add_argument("--fan2d", action="store_true",
                     help="the 2D fan-beam configuration (256^2, 360 views x 512 bins) with its CPU baseline")
+    ap.add_argument("--refcalls", action="store_true",
+                    help="the 256^3 cone-beam workload through the _C calls as the reference's Python makes them")
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven 2D scans: one persistent launch (autotuned) vs 64 sequential launches")
     ap.add_argument("--map", action="store_true",
@@ -416,6 +458,8 @@ def main(argv=None) -> int:
         return batched_bench(a, dev)
     if a.fan2d:
         return fan2d_bench(a, dev)
+    if a.refcalls:
+        return refcalls_bench(a, dev)
     if a.copy_calibration:
         x = torch.ones(1 << 28, device=dev)  # 1 GiB of float32
         y = torch.empty_like(x)
