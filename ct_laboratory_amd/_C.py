@@ -412,6 +412,89 @@ def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, 
 
 
 # ---------------------------------------------------------------------------
+# FP64 values over the same traversal (no reference counterpart; derivative checks)
+# ---------------------------------------------------------------------------
+
+
+def _check_f64_values(name: str, values: torch.Tensor, src, dst, M_inv, b) -> None:
+    _require(isinstance(values, torch.Tensor) and values.is_cuda and values.dtype == torch.float64
+             and values.is_contiguous(), f"{name}: expected a contiguous float64 CUDA (HIP) tensor")
+    _check_f32(name, src, dst, M_inv, b)
+    _require(values.device == src.device, f"{name}: all tensors must be on one device")
+
+
+def _out64(name: str, out, shape, device) -> torch.Tensor:
+    shape = tuple(int(x) for x in shape)
+    if out is None:
+        return torch.empty(shape, device=device, dtype=torch.float64)
+    _require(out.dtype == torch.float64 and out.is_contiguous() and tuple(out.shape) == shape and out.device == device,
+             f"{name}: out must be a contiguous float64 tensor of shape {shape} on the inputs' device")
+    return out
+
+
+def forward_project_3d_on_the_fly_f64(volume, src, dst, M_inv, b, *, mode: int = N.MODE_OTF, out=None):
+    """[B, n_ray] float64 line integrals over the segments of
+    forward_project_3d_on_the_fly_cuda (double values and sums)."""
+    name = "forward_project_3d_on_the_fly_f64"
+    _check_f64_values(name, volume, src, dst, M_inv, b)
+    batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
+    n_ray = _rays(name, src, dst, 3)
+    _affine(name, M_inv, b, 3)
+    out = _out64(name, out, (batch, n_ray), volume.device)
+    with torch.cuda.device(volume.device):
+        N.call("ctlab_forward_project_3d_otf_f64", N.ptr(volume), batch, nx, ny, nz, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), mode, N.ptr(out), N.stream_of(volume))
+    return out
+
+
+def back_project_3d_on_the_fly_f64(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_OTF, out=None):
+    """[B, n_x, n_y, n_z] float64 back projection, the exact transpose of
+    forward_project_3d_on_the_fly_f64."""
+    name = "back_project_3d_on_the_fly_f64"
+    _check_f64_values(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    _affine(name, M_inv, b, 3)
+    out = _out64(name, out, (batch, n_x, n_y, n_z), sino.device)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_otf_f64", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
+               N.ptr(b), int(n_x), int(n_y), int(n_z), mode, 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+def forward_project_2d_on_the_fly_f64(img, src, dst, M_inv, b, *, mode: int = N.MODE_OTF,
+                                      zero_rule: int = N.ZERO_THRESH, out=None):
+    """[B, n_ray] float64 line integrals over the segments of
+    forward_project_2d_on_the_fly_cuda."""
+    name = "forward_project_2d_on_the_fly_f64"
+    _check_f64_values(name, img, src, dst, M_inv, b)
+    batch, (nr, nc) = _vol_dims(name, img, 2)
+    n_ray = _rays(name, src, dst, 2)
+    _affine(name, M_inv, b, 2)
+    out = _out64(name, out, (batch, n_ray), img.device)
+    with torch.cuda.device(img.device):
+        N.call("ctlab_forward_project_2d_otf_f64", N.ptr(img), batch, nr, nc, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), mode, zero_rule, N.ptr(out), N.stream_of(img))
+    return out
+
+
+def back_project_2d_on_the_fly_f64(sino, src, dst, M_inv, b, n_row, n_col, *, mode: int = N.MODE_OTF,
+                                   zero_rule: int = N.ZERO_THRESH, out=None):
+    """[B, n_row, n_col] float64 back projection, the exact transpose of
+    forward_project_2d_on_the_fly_f64."""
+    name = "back_project_2d_on_the_fly_f64"
+    _check_f64_values(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 2) == n_ray, f"{name}: sino and src disagree on n_ray")
+    _affine(name, M_inv, b, 2)
+    out = _out64(name, out, (batch, n_row, n_col), sino.device)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_2d_otf_f64", N.ptr(sino), batch, n_ray, N.ptr(src), N.ptr(dst), N.ptr(M_inv),
+               N.ptr(b), int(n_row), int(n_col), mode, zero_rule, 0, N.ptr(out), N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Atomic-free back projection of flat-panel view stacks (no reference counterpart)
 # ---------------------------------------------------------------------------
 

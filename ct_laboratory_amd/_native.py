@@ -47,6 +47,10 @@ _SIGS = {
                                   _P],
     "ctlab_forward_project_2d_otf": [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _P, _P],
     "ctlab_back_project_2d_otf": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _INT, _INT, _INT, _P, _P],
+    "ctlab_forward_project_3d_otf_f64": [_P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _P, _P],
+    "ctlab_back_project_3d_otf_f64": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _INT, _P, _P],
+    "ctlab_forward_project_2d_otf_f64": [_P, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _P, _P],
+    "ctlab_back_project_2d_otf_f64": [_P, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _INT, _INT, _INT, _P, _P],
     "ctlab_trace_3d": [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
     "ctlab_trace_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _INT, _I64, _P, _P, _P, _P],
     "ctlab_compute_intersections_2d": [_I64, _I64, _P, _P, _I64, _P, _P, _INT, _I64, _P, _P],

@@ -66,8 +66,8 @@ def forward_project_2d_on_the_fly_hip(image, M, b, src, dst, *, semantics: str =
     """On-the-fly 2D forward projection: [B, n_ray] or [n_ray].  zero_rule
     (None: the rule's own) selects which axes have no crossings."""
     zr = _zero_rule(semantics) if zero_rule is None else int(zero_rule)
-    out = _C.forward_project_2d_on_the_fly_cuda(image, src, dst, _M_inv(M), _b(b), mode=semantics_mode(semantics),
-                                                zero_rule=zr)
+    fn = _C.forward_project_2d_on_the_fly_f64 if image.dtype == torch.float64 else _C.forward_project_2d_on_the_fly_cuda
+    out = fn(image, src, dst, _M_inv(M), _b(b), mode=semantics_mode(semantics), zero_rule=zr)
     return out.squeeze(0) if image.dim() == 2 else out
 
 
@@ -75,8 +75,8 @@ def back_project_2d_on_the_fly_hip(sinogram, M, b, src, dst, n_row, n_col, *, se
                                    zero_rule=None):
     """On-the-fly 2D back projection, the exact transpose of the forward."""
     zr = _zero_rule(semantics) if zero_rule is None else int(zero_rule)
-    out = _C.back_project_2d_on_the_fly_cuda(sinogram, src, dst, _M_inv(M), _b(b), n_row, n_col,
-                                             mode=semantics_mode(semantics), zero_rule=zr)
+    fn = _C.back_project_2d_on_the_fly_f64 if sinogram.dtype == torch.float64 else _C.back_project_2d_on_the_fly_cuda
+    out = fn(sinogram, src, dst, _M_inv(M), _b(b), n_row, n_col, mode=semantics_mode(semantics), zero_rule=zr)
     return out.squeeze(0) if sinogram.dim() == 1 else out
 
 

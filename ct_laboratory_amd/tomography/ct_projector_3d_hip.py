@@ -127,11 +127,12 @@ def forward_project_3d_hip(volume, tvals, src, dst, M, b, *, semantics: str = "o
     """Line integrals [B, n_ray] (or [n_ray] for a 3D volume) (ct_projector_3d_cuda.py:112-160).
 
     ``tvals``: float table, (t_uint16, t_start, t_scale) tuple, or None/empty for
-    the on-the-fly traversal."""
+    the on-the-fly traversal.  A float64 volume (on-the-fly only) is projected
+    with double values over the same segments."""
     M, b = _collapse(M, b)
-    _check_inputs(volume=volume, src=src, dst=dst, M=M, b=b)
     if _empty_or_none(tvals):
         return forward_project_3d_on_the_fly_hip(volume, M, b, src, dst, semantics=semantics, det_shape=det_shape)
+    _check_inputs(volume=volume, src=src, dst=dst, M=M, b=b)
     M_inv = _prepare_M_inv(M)
     if isinstance(tvals, (tuple, list)):
         t_u16, t_start, t_scale = tvals
@@ -157,10 +158,10 @@ def back_project_3d_hip(sinogram, tvals, src, dst, M, b, n_x, n_y, n_z, *, seman
     With no table, ``brick_plan`` (from ``ConeBeamPlan``) selects the LDS-brick
     back projector; otherwise the atomic one runs."""
     M, b = _collapse(M, b)
-    _check_inputs(sinogram=sinogram)
     if _empty_or_none(tvals):
         return back_project_3d_on_the_fly_hip(sinogram, src, dst, M, b, n_x, n_y, n_z, semantics=semantics,
                                               det_shape=det_shape, brick_plan=brick_plan)
+    _check_inputs(sinogram=sinogram)
     M_inv = _prepare_M_inv(M)
     if isinstance(tvals, (tuple, list)):
         t_u16, t_start, t_scale = tvals
@@ -186,12 +187,18 @@ def back_project_3d_compressed_hip(sinogram, t_uint16, t_start, t_scale, M, b, s
 
 
 def forward_project_3d_on_the_fly_hip(volume, M, b, src, dst, *, semantics: str = "otf", det_shape=None):
-    """On-the-fly forward projection (ct_projector_3d_cuda.py:253-287)."""
+    """On-the-fly forward projection (ct_projector_3d_cuda.py:253-287).  A
+    float64 volume gives float64 line integrals over the same segments."""
     M, b = _collapse(M, b)
     M, b = _f32c(M), _f32c(b)
-    _check_inputs(volume=volume, src=src, dst=dst)
-    out = _C.forward_project_3d_on_the_fly_cuda(volume, src, dst, _prepare_M_inv(M), b,
-                                                mode=semantics_mode(semantics), det_shape=det_shape)
+    _check_inputs(src=src, dst=dst)
+    if volume.dtype == torch.float64:
+        out = _C.forward_project_3d_on_the_fly_f64(volume.contiguous(), src, dst, _prepare_M_inv(M), b,
+                                                   mode=semantics_mode(semantics))
+    else:
+        _check_inputs(volume=volume)
+        out = _C.forward_project_3d_on_the_fly_cuda(volume, src, dst, _prepare_M_inv(M), b,
+                                                    mode=semantics_mode(semantics), det_shape=det_shape)
     return out.squeeze(0) if volume.dim() == 3 else out
 
 
@@ -202,9 +209,13 @@ def back_project_3d_on_the_fly_hip(sinogram, src, dst, M, b, n_x, n_y, n_z, *, s
     here once)."""
     M, b = _collapse(M, b)
     M, b = _f32c(M), _f32c(b)
-    _check_inputs(sinogram=sinogram, src=src, dst=dst)
+    _check_inputs(src=src, dst=dst)
     mode = semantics_mode(semantics)
     M_inv = _prepare_M_inv(M)
+    if sinogram.dtype == torch.float64:  # double values over the same segments
+        out = _C.back_project_3d_on_the_fly_f64(sinogram.contiguous(), src, dst, M_inv, b, n_x, n_y, n_z, mode=mode)
+        return out.squeeze(0) if sinogram.dim() == 1 else out
+    _check_inputs(sinogram=sinogram)
     if brick_plan is not None and det_shape is not None:
         out = _C.back_project_3d_brick(sinogram, src, dst, det_shape, brick_plan, M_inv, b, n_x, n_y, n_z, mode=mode)
     else:

@@ -100,6 +100,28 @@ int ctlab_back_project_2d_otf(const float* sino, int64_t batch, int64_t n_ray, c
                               ctlab_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* FP64 values over the same traversal (no counterpart in the reference;   */
+/* for derivative checks): the segments, voxel indices and float32 lengths */
+/* are those of the entry points above, the volume / image, sinogram and   */
+/* sums are double.  Forward: one fused multiply-add per segment in        */
+/* segment order; back: double atomic adds (out zeroed first unless        */
+/* accumulate != 0).  One ray per lane in storage order.                   */
+/* ---------------------------------------------------------------------- */
+int ctlab_forward_project_3d_otf_f64(const double* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
+                                     const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                     const float* b, int mode, double* out, ctlab_stream_t stream);
+int ctlab_back_project_3d_otf_f64(const double* sino, int64_t batch, int64_t n_ray, const float* src,
+                                  const float* dst, const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
+                                  int64_t n_z, int mode, int accumulate, double* out, ctlab_stream_t stream);
+int ctlab_forward_project_2d_otf_f64(const double* image, int64_t batch, int64_t n_row, int64_t n_col,
+                                     const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                     const float* b, int mode, int zero_rule, double* out, ctlab_stream_t stream);
+int ctlab_back_project_2d_otf_f64(const double* sino, int64_t batch, int64_t n_ray, const float* src,
+                                  const float* dst, const float* M_inv, const float* b, int64_t n_row,
+                                  int64_t n_col, int mode, int zero_rule, int accumulate, double* out,
+                                  ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Per-ray segment lists (parity instrument; no counterpart in the         */
 /* reference): for each ray the flat voxel index and length of every        */
 /* segment with length > 0, in traversal order.  idx/len are [n_ray, cap];  */
