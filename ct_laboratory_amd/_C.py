@@ -26,7 +26,11 @@ plan is cached per (ray set, grid, M_inv and b values).
 Likewise a 2D crossing table that compute_intersections_2d returned, passed
 back untouched with the same rays, grid and map, is projected on the fly with
 its own rule (the same segments in the same order: a bit-identical forward, and
-a back projection over LDS tiles instead of per-segment atomics).
+a back projection over LDS tiles instead of per-segment atomics).  Any other
+full 2D table (a copy, one loaded from disk) is compared once, bit for bit,
+with the table computed here for its rays, grid and map, and projected the same
+way when they are equal; the comparison costs one table computation and one
+synchronisation per table version.
 
 ``AUTO_LAYOUT = False`` (or CTLAB_AUTO_LAYOUT=0 in the environment) turns both
 off.
@@ -231,11 +235,16 @@ def compute_intersections_2d(n_row, n_col, src, dst, M_inv, b, *, zero_rule: int
     n_ray = _rays(name, src, dst, 2)
     _affine(name, M_inv, b, 2)
     K = int(n_row) + int(n_col) + 2 if K is None else int(K)
+    out = _table2d(n_row, n_col, src, dst, M_inv, b, zero_rule, K, n_ray)
+    _register_table2d(out, n_row, n_col, src, dst, M_inv, b, zero_rule, K)
+    return out
+
+
+def _table2d(n_row, n_col, src, dst, M_inv, b, zero_rule, K, n_ray):
     out = torch.empty((n_ray, K), device=src.device, dtype=torch.float32)
     with torch.cuda.device(src.device):
         N.call("ctlab_compute_intersections_2d", int(n_row), int(n_col), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv),
                N.ptr(b), zero_rule, K, N.ptr(out), N.stream_of(src))
-    _register_table2d(out, n_row, n_col, src, dst, M_inv, b, zero_rule, K)
     return out
 
 
@@ -652,41 +661,77 @@ def _plan_of(src, dst, layout, M_inv, b, shape):
 # ---------------------------------------------------------------------------
 
 # id(table) -> (weak reference, version, (n_row, n_col, src ref, src version, dst ref, dst version,
-# M_inv values, b values, zero rule)) of full tables compute_intersections_2d returned
+# M_inv values, b values), zero rule) of full tables whose rule is known for those rays, grid and map:
+# the tables compute_intersections_2d returned, and tables found equal to one it computes (zero rule
+# None: found different, consumed as given)
 _TABLES2D: dict = {}
+# id(table) -> (weak reference, number of comparisons that found it different)
+_TABLE2D_MISSES: dict = {}
+_MAX_TABLE2D_MISSES = 2
+
+
+def _geometry2d(n_row, n_col, src, dst, M_inv, b):
+    return (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
+            tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()))
+
+
+def _prune(cache):
+    if len(cache) > 64:
+        for k in [k for k, e in cache.items() if e[0]() is None]:
+            del cache[k]
 
 
 def _register_table2d(table, n_row, n_col, src, dst, M_inv, b, zero_rule, K):
     if K != int(n_row) + int(n_col) + 2 or _capturing():
         return
-    record = (weakref.ref(table), table._version,
-              (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
-               tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()), int(zero_rule)))
+    record = (weakref.ref(table), table._version, _geometry2d(n_row, n_col, src, dst, M_inv, b),
+              None if zero_rule is None else int(zero_rule))
     with _CACHE_LOCK:
-        if len(_TABLES2D) > 64:
-            for k in [k for k, e in _TABLES2D.items() if e[0]() is None]:
-                del _TABLES2D[k]
+        _prune(_TABLES2D)
         _TABLES2D[id(table)] = record
 
 
 def _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col):
-    """The zero rule of a full table that compute_intersections_2d returned for
-    these very rays, grid and map and that nothing has written to since: its
-    segments are exactly those of the on-the-fly torch rule with that zero rule,
-    in the same order.  None otherwise (the table is consumed as given)."""
+    """The zero rule with which the on-the-fly torch rule gives exactly the
+    segments of this table, in the same order, for these rays, grid and map;
+    None when the table is to be consumed as given.
+
+    A full table that compute_intersections_2d returned for these very rays,
+    grid and map, and that nothing has written to since, has its rule on record.
+    Any other full table (a copy, one loaded from disk, one written in place) is
+    compared once, bit for bit, with the table computed here for the same rays,
+    grid and map, and the outcome is kept until the table, the rays or the map
+    change.  A table found different twice is not compared again."""
     if not AUTO_LAYOUT or _capturing():
         return None
     with _CACHE_LOCK:
         e = _TABLES2D.get(id(tvals))
-    if e is None or e[0]() is not tvals or tvals._version != e[1]:
+    if e is not None and e[0]() is tvals and tvals._version == e[1]:
+        nr, nc, sref, sver, dref, dver, mv, bv = e[2]
+        if ((nr, nc) == (int(n_row), int(n_col)) and sref() is src and dref() is dst and src._version == sver and
+                dst._version == dver and tuple(M_inv.flatten().tolist()) == mv and
+                tuple(b.flatten().tolist()) == bv):
+            return e[3]
+    return _compare_table2d(tvals, src, dst, M_inv, b, n_row, n_col)
+
+
+def _compare_table2d(tvals, src, dst, M_inv, b, n_row, n_col):
+    K = int(n_row) + int(n_col) + 2
+    if tvals.shape[1] != K or not tvals.is_contiguous() or tvals.device != src.device:
         return None
-    nr, nc, sref, sver, dref, dver, mv, bv, zr = e[2]
-    if (nr, nc) != (int(n_row), int(n_col)) or sref() is not src or dref() is not dst:
+    with _CACHE_LOCK:
+        m = _TABLE2D_MISSES.get(id(tvals))
+    misses = m[1] if (m is not None and m[0]() is tvals) else 0
+    if misses >= _MAX_TABLE2D_MISSES:
         return None
-    if src._version != sver or dst._version != dver:
-        return None
-    if tuple(M_inv.flatten().tolist()) != mv or tuple(b.flatten().tolist()) != bv:
-        return None
+    own = _table2d(n_row, n_col, src, dst, M_inv, b, N.ZERO_THRESH, K, int(tvals.shape[0]))
+    zr = N.ZERO_THRESH if torch.equal(own, tvals) else None
+    del own
+    _register_table2d(tvals, n_row, n_col, src, dst, M_inv, b, zr, K)
+    if zr is None:
+        with _CACHE_LOCK:
+            _prune(_TABLE2D_MISSES)
+            _TABLE2D_MISSES[id(tvals)] = (weakref.ref(tvals), misses + 1)
     return zr
 
 

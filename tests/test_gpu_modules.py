@@ -379,3 +379,26 @@ def test_fanbeam_full_size_config(dev):
         lhs = float(got.astype(np.float64) @ y.astype(np.float64))
         rhs = float((_np(img).astype(np.float64) * bk).sum())
         assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_copied_table_runs_on_the_fly(dev):
+    """A 2D module whose table is a copy (as after loading it from disk) is
+    compared once with the table computed for its rays, grid and map; being
+    equal, it is projected on the fly with the table's rule: the same results
+    as the module's own table (the back projection on LDS tiles)."""
+    from ct_laboratory_amd import _C
+    from ct_laboratory_amd import _native as N
+
+    n = 96
+    proj = FanBeam2DProjector(n, n, 30, 140, 300.0, 450.0, det_spacing=1.0, pixel_size=1.0, device=dev)
+    img = torch.from_numpy(ellipse_phantom_2d(n, n, 1.0, seed=6)).to(dev)
+    y = torch.from_numpy(np.random.default_rng(10).uniform(0, 1, proj.src.shape[0]).astype(np.float32)).to(dev)
+    own_fwd, own_back = proj.forward_project(img), proj.back_project(y)
+    proj.tvals = proj.tvals.clone()
+    assert proj._rule()[0] is proj.tvals
+    assert torch.equal(proj.forward_project(img), own_fwd)
+    assert rel_err(_np(proj.back_project(y)), _np(own_back)) < 1e-6
+    assert _C._TABLES2D[id(proj.tvals)][3] == N.ZERO_THRESH
+    x = img.clone().requires_grad_(True)
+    proj(x).backward(y)
+    assert rel_err(_np(x.grad), _np(own_back)) < 1e-6

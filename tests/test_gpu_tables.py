@@ -58,16 +58,39 @@ def test_table_consumer_equals_on_the_fly(dev, nd):
         b_otf = _C.back_project_3d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH)
     else:
         own = _C.compute_intersections_2d(*shape, ts, td, tm, tb, zero_rule=zr)
-        tab = own.clone()  # a copy is consumed as given: the table kernels run
-        f_tab = _C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb)
+        tab = own.clone()
+        _C.AUTO_LAYOUT = False  # the table kernels
+        try:
+            f_tab = _C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb)
+            b_tab = _C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape)
+        finally:
+            _C.AUTO_LAYOUT = True
         f_otf = _C.forward_project_2d_on_the_fly_cuda(vol, ts, td, tm, tb, mode=N.MODE_TORCH, zero_rule=zr)
-        b_tab = _C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape)
         b_otf = _C.back_project_2d_on_the_fly_cuda(sino, ts, td, tm, tb, *shape, mode=N.MODE_TORCH, zero_rule=zr)
         # the table as returned runs on the fly with its own rule
         assert torch.equal(_C.forward_project_2d_cuda(vol, own, ts, td, tm, tb), f_otf)
         assert torch.equal(_C.back_project_2d_cuda(sino, own, ts, td, tm, tb, *shape), b_otf)
+        # a copy is compared once with the table computed here, found equal and run on the fly
+        assert _C._table2d_rule(tab, ts, td, tm, tb, *shape) == zr
+        assert torch.equal(_C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb), f_otf)
+        assert torch.equal(_C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape), b_otf)
+        r = int(torch.nonzero(torch.isfinite(own[:, 1]))[0])
         with torch.no_grad():
-            own[0, 0] = float("inf")  # written: consumed as given again
+            own[r, 0] = float("inf")  # written: differs from the computed table, consumed as given
+        assert _C._table2d_rule(own, ts, td, tm, tb, *shape) is None
+        _C.AUTO_LAYOUT = False
+        try:
+            f_edit = _C.forward_project_2d_cuda(vol, own, ts, td, tm, tb)
+        finally:
+            _C.AUTO_LAYOUT = True
+        assert torch.equal(_C.forward_project_2d_cuda(vol, own, ts, td, tm, tb), f_edit)
+        # each version is compared once; after two differing versions the table is not compared again
+        with torch.no_grad():
+            own[r, 1] = float("inf")
+        assert _C._table2d_rule(own, ts, td, tm, tb, *shape) is None
+        assert _C._TABLE2D_MISSES[id(own)][1] == 2
+        with torch.no_grad():
+            own.copy_(_C.compute_intersections_2d(*shape, ts, td, tm, tb, zero_rule=zr))
         assert _C._table2d_rule(own, ts, td, tm, tb, *shape) is None
     assert torch.equal(f_tab, f_otf)
     assert rel_err(b_tab.cpu().numpy(), b_otf.cpu().numpy()) < 1e-6
@@ -89,8 +112,18 @@ def test_table_projectors_vs_oracle(dev, nd):
         f = _C.forward_project_3d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
         bk = _C.back_project_3d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
     else:
-        f = _C.forward_project_2d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
-        bk = _C.back_project_2d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
+        _C.AUTO_LAYOUT = False  # the table kernels
+        try:
+            f = _C.forward_project_2d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
+            bk = _C.back_project_2d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
+        finally:
+            _C.AUTO_LAYOUT = True
+        # the oracle's table equals the one computed here: the same call runs on the fly, on tiles
+        assert _C._table2d_rule(tt, ts, td, tm, tb, *shape) == N.ZERO_THRESH
+        f2 = _C.forward_project_2d_cuda(_t(vol, dev), tt, ts, td, tm, tb)
+        bk2 = _C.back_project_2d_cuda(_t(sino, dev), tt, ts, td, tm, tb, *shape)
+        assert torch.equal(f2, f)
+        assert rel_err(bk2.cpu().numpy(), O.back_tvals(sino, tab, shape, minv, b, src, dst)) < 1e-5
     assert rel_err(f.cpu().numpy(), O.forward_tvals(vol, tab, minv, b, src, dst)) < 1e-5
     assert rel_err(bk.cpu().numpy(), O.back_tvals(sino, tab, shape, minv, b, src, dst)) < 1e-5
 
