@@ -668,6 +668,7 @@ _TABLES2D: dict = {}
 # id(table) -> (weak reference, number of comparisons that found it different)
 _TABLE2D_MISSES: dict = {}
 _MAX_TABLE2D_MISSES = 2
+_TABLE2D_COMPARE_BYTES = 256 << 20  # table bytes computed per comparison chunk
 
 
 def _geometry2d(n_row, n_col, src, dst, M_inv, b):
@@ -724,9 +725,17 @@ def _compare_table2d(tvals, src, dst, M_inv, b, n_row, n_col):
     misses = m[1] if (m is not None and m[0]() is tvals) else 0
     if misses >= _MAX_TABLE2D_MISSES:
         return None
-    own = _table2d(n_row, n_col, src, dst, M_inv, b, N.ZERO_THRESH, K, int(tvals.shape[0]))
-    zr = N.ZERO_THRESH if torch.equal(own, tvals) else None
-    del own
+    # in chunks of rays, so that a large table is not held twice
+    n_ray = int(tvals.shape[0])
+    step = max(1, _TABLE2D_COMPARE_BYTES // (4 * K))
+    same = True
+    for r0 in range(0, n_ray, step):
+        r1 = min(n_ray, r0 + step)
+        own = _table2d(n_row, n_col, src[r0:r1], dst[r0:r1], M_inv, b, N.ZERO_THRESH, K, r1 - r0)
+        if not torch.equal(own, tvals[r0:r1]):
+            same = False
+            break
+    zr = N.ZERO_THRESH if same else None
     _register_table2d(tvals, n_row, n_col, src, dst, M_inv, b, zr, K)
     if zr is None:
         with _CACHE_LOCK:

@@ -70,8 +70,18 @@ def test_table_consumer_equals_on_the_fly(dev, nd):
         # the table as returned runs on the fly with its own rule
         assert torch.equal(_C.forward_project_2d_cuda(vol, own, ts, td, tm, tb), f_otf)
         assert torch.equal(_C.back_project_2d_cuda(sino, own, ts, td, tm, tb, *shape), b_otf)
-        # a copy is compared once with the table computed here, found equal and run on the fly
-        assert _C._table2d_rule(tab, ts, td, tm, tb, *shape) == zr
+        # a copy is compared once with the table computed here, found equal and run on the fly;
+        # compared in chunks of rays, a difference in the last chunk is found
+        saved = _C._TABLE2D_COMPARE_BYTES
+        _C._TABLE2D_COMPARE_BYTES = 4 * tab.shape[1] * 300
+        try:
+            late = tab.clone()
+            with torch.no_grad():
+                late[-1, 0] = float("nan")
+            assert _C._table2d_rule(late, ts, td, tm, tb, *shape) is None
+            assert _C._table2d_rule(tab, ts, td, tm, tb, *shape) == zr
+        finally:
+            _C._TABLE2D_COMPARE_BYTES = saved
         assert torch.equal(_C.forward_project_2d_cuda(vol, tab, ts, td, tm, tb), f_otf)
         assert torch.equal(_C.back_project_2d_cuda(sino, tab, ts, td, tm, tb, *shape), b_otf)
         r = int(torch.nonzero(torch.isfinite(own[:, 1]))[0])
