@@ -4,8 +4,10 @@
 // One lane traces one ray (siddon_traverse.h).  The forward kernel gathers
 // volume values along the ray and writes one line integral per lane; the back
 // kernel scatters sino*length with float atomic adds (global_atomic_add_f32,
-// no return value).  blockIdx.y is the batch index, so the traversal of a ray
-// is repeated per batch member but each lane's gathers stay in one volume.
+// no return value).  In the forward, blockIdx.y is a group of up to NB batch
+// members: a lane traces its ray once and gathers from each member's volume,
+// so the traversal arithmetic (which bounds the kernel) is shared by the
+// group.  In the back projection blockIdx.y is the batch index.
 #include "ctlab_common.h"
 #include "siddon_traverse.h"
 
@@ -82,35 +84,68 @@ __device__ __forceinline__ void load_ray(const float* __restrict__ src, const fl
     }
 }
 
-template <int ND, int MODE>
+template <int ND, int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict__ vol, float* __restrict__ out,
                                                         RaySched sched, int64_t vol_elems, Dims<ND> dims,
                                                         const float* __restrict__ src,
                                                         const float* __restrict__ dst,
                                                         const float* __restrict__ minv,
-                                                        const float* __restrict__ b, int zero_rule) {
+                                                        const float* __restrict__ b, int zero_rule, int batch) {
     const int64_t r = ray_of_lane(sched);
     if (r < 0) return;
-    const int64_t bi = blockIdx.y;
+    const int b0 = (int)blockIdx.y * NB;
     const int64_t n_ray = sched.n_ray;
     const Grid<ND> g = load_grid<ND>(dims.n, minv, b, zero_rule);
     float s[ND], d[ND];
     load_ray<ND>(src, dst, r, s, d);
-    const float* __restrict__ v = vol + bi * vol_elems;
+    // Members past the end of the batch (last group) read the last member's
+    // volume and are not written.
+    const float* __restrict__ v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) v[k] = vol + (int64_t)min(b0 + k, batch - 1) * vol_elems;
     // The gather of a segment is consumed at the next visited segment, so its
-    // memory latency overlaps the next crossing and midpoint computation.  The
-    // sum is formed in the same order (acc = fma(value, len, acc) per segment).
-    float acc = 0.0f, pv = 0.0f, pl = 0.0f;
+    // memory latency overlaps the next crossing and midpoint computation.  Each
+    // member's sum is formed in segment order (acc = fma(value, len, acc)),
+    // whatever NB is.
+    float acc[NB], pv[NB], pl = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc[k] = pv[k] = 0.0f;
     auto gather = [&](uint32_t idx, float len) {
-        acc = __builtin_fmaf(pv, pl, acc);
-        pv = v[idx];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            acc[k] = __builtin_fmaf(pv[k], pl, acc[k]);
+            pv[k] = v[k][idx];
+        }
         pl = len;
     };
     if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, gather);
     else
         trace_ray<ND, MODE, false>(g, s, d, gather);
-    out[bi * n_ray + r] = __builtin_fmaf(pv, pl, acc);
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+        if (b0 + k < batch) out[(int64_t)(b0 + k) * n_ray + r] = __builtin_fmaf(pv[k], pl, acc[k]);
+}
+
+// Batch members per trace.  Four share a trace only while their four volumes
+// stay well inside the 256 MB Infinity Cache (at 256^3, four volumes gathered
+// together ran slower per volume than two); otherwise two.
+constexpr int64_t kGroup4Bytes = 128ll << 20;
+
+template <int ND, int MODE>
+static void launch_forward_otf(const float* vol, float* out, const RaySched& sched, int64_t V, Dims<ND> dims,
+                               const float* src, const float* dst, const float* minv, const float* b, int zero_rule,
+                               int64_t batch, hipStream_t stream) {
+    const int bt = (int)batch;
+    if (batch >= 4 && 4 * V * (int64_t)sizeof(float) <= kGroup4Bytes)
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE, 4>), dim3(sched.n_blocks, (unsigned)((batch + 3) / 4)),
+                           dim3(kBlock), 0, stream, vol, out, sched, V, dims, src, dst, minv, b, zero_rule, bt);
+    else if (batch >= 2)
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE, 2>), dim3(sched.n_blocks, (unsigned)((batch + 1) / 2)),
+                           dim3(kBlock), 0, stream, vol, out, sched, V, dims, src, dst, minv, b, zero_rule, bt);
+    else
+        hipLaunchKernelGGL((k_forward_otf<ND, MODE, 1>), dim3(sched.n_blocks, 1u), dim3(kBlock), 0, stream, vol,
+                           out, sched, V, dims, src, dst, minv, b, zero_rule, bt);
 }
 
 template <int ND, int MODE>
@@ -234,13 +269,12 @@ static int forward_otf(const char* fn, const float* vol, int64_t batch, const in
     CTLAB_REQUIRE(vol && out, "%s: volume/out is null", fn);
     int64_t V = 1;
     for (int a = 0; a < ND; ++a) V *= n[a];
-    dim3 grid(sched.n_blocks, (unsigned)batch);
     if (mode == CTLAB_MODE_TORCH)
-        hipLaunchKernelGGL((k_forward_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, vol, out, sched, V,
-                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+        launch_forward_otf<ND, MODE_TORCH>(vol, out, sched, V, make_dims<ND>(n), src, dst, minv, b, zero_rule, batch,
+                                           stream);
     else
-        hipLaunchKernelGGL((k_forward_otf<ND, MODE_OTF>), grid, dim3(kBlock), 0, stream, vol, out, sched, V,
-                           make_dims<ND>(n), src, dst, minv, b, zero_rule);
+        launch_forward_otf<ND, MODE_OTF>(vol, out, sched, V, make_dims<ND>(n), src, dst, minv, b, zero_rule, batch,
+                                         stream);
     return check_launch(fn);
 }
 

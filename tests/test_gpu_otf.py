@@ -193,6 +193,35 @@ def test_detector_tiled_schedule(dev, mname, mode):
         _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, e3, b, det_shape=(71, 37))
 
 
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_forward_batch_groups(dev, mname, mode):
+    """A forward launch traces each ray once for a group of up to four batch
+    members: every member's line integrals equal its own batch-1 projection bit
+    for bit, for batches of 2, 3 and 5 (ragged last group), with and without a
+    detector layout, in 3D and 2D."""
+    n = 24
+    src, dst = build_conebeam_3d_geometry(5, 21, 18, 200.0, 300.0, det_spacing=2.5)
+    b = _t(np.full(3, -(n - 1) / 2, np.float32), dev)
+    e3 = _t(np.eye(3), dev)
+    ts, td = src.to(dev).contiguous(), dst.to(dev).contiguous()
+    rng = np.random.default_rng(12)
+    vol = _t(rng.uniform(-1, 1, (5, n, n, n)), dev)
+    for det in (None, (18, 21)):
+        one = torch.cat([_C.forward_project_3d_on_the_fly_cuda(vol[i], ts, td, e3, b, mode=mode, det_shape=det)
+                         for i in range(5)])
+        for nb in (2, 3, 5):
+            got = _C.forward_project_3d_on_the_fly_cuda(vol[:nb].contiguous(), ts, td, e3, b, mode=mode,
+                                                        det_shape=det)
+            assert torch.equal(got, one[:nb])
+    shape, M, minv, b2, s = random_geometry(rng, 2, max_n=30)
+    s2, d2 = random_rays(rng, shape, s, 3000, "far")
+    img = _t(rng.uniform(-1, 1, (5,) + shape), dev)
+    args = (_t(s2, dev), _t(d2, dev), _t(minv, dev), _t(b2, dev))
+    one = torch.cat([_C.forward_project_2d_on_the_fly_cuda(img[i], *args, mode=mode) for i in range(5)])
+    for nb in (2, 3, 5):
+        assert torch.equal(_C.forward_project_2d_on_the_fly_cuda(img[:nb].contiguous(), *args, mode=mode), one[:nb])
+
+
 def test_empty_and_zero(dev):
     z = torch.zeros((0, 3), device=dev)
     vol = torch.ones((4, 4, 4), device=dev)
