@@ -62,8 +62,9 @@ const char* ctlab_last_error(void);
  * det_rows/det_cols (0 = none) declare that the rays are ordered
  * [view][det_rows][det_cols] with columns fastest, as the cone-beam builder
  * orders them (conebeam_projector_3d.py:84-96); the kernel then schedules
- * each wavefront over 64 consecutive detector rows of one column.  Results do
- * not depend on it. */
+ * each wavefront over 16 consecutive detector rows of 4 adjacent columns.
+ * Results do not depend on it.  With batch >= 2 each ray is traced once for a
+ * group of 2 or 4 volumes (same results as one volume at a time). */
 int ctlab_forward_project_3d_otf(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
                                  const float* src, const float* dst, int64_t n_ray, const float* M_inv,
                                  const float* b, int mode, int64_t det_rows, int64_t det_cols, float* out,
