@@ -4,8 +4,11 @@
 Workload: 256^3 float32 volume at 1.0 mm (20 seeded ellipsoids); 720 views over
 360 deg; 384 x 384 flat detector at 1.5 mm pitch; SID 1000 mm, SDD 1500 mm
 (106,168,320 rays).  One step = one forward projection of the whole volume plus
-one back projection of the resulting sinogram, on one GPU, inputs resident in
-HBM.  Prints one JSON line.
+one back projection of a dense sinogram (y ~ N(0,1), seed 2: every ray nonzero,
+as a MAP residual is), on one GPU, inputs resident in HBM.  The back projectors
+skip rays whose value is exactly 0, so a forward projection of the phantom
+(65% zero rays at this size) would time a sparse workload; it is reported
+beside the dense time, never as it.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--mode torch|otf] [--no-cpu]
 
@@ -497,13 +500,17 @@ def main(argv=None) -> int:
                                             mode=mode)
         return _C.back_project_3d_on_the_fly_cuda(y, src, dst, minv, b, n, n, n, mode=mode, det_shape=det)
 
+    # dense sinogram for the back projection: every ray carries a nonzero value
+    y_dense = torch.from_numpy(np.random.default_rng(2).standard_normal(n_ray, dtype=np.float32)).to(dev)
+    n_zero = int((y_dense == 0).sum().item())
+
     def step(ev=None):
         if ev is not None:
             ev[0].record()
         sino = _C.forward_project_3d_on_the_fly_cuda(vol, src, dst, minv, b, mode=mode, det_shape=det)
         if ev is not None:
             ev[1].record()
-        bk = back(sino)
+        bk = back(y_dense)
         if ev is not None:
             ev[2].record()
         return sino, bk
@@ -526,13 +533,21 @@ def main(argv=None) -> int:
 
     adjoint = None
     if not a.no_adjoint:
-        # Config check: <A x, y> vs <x, A^T y> with y ~ N(0, 1) (seed 2), FP64 dot products.
-        y = torch.from_numpy(np.random.default_rng(2).standard_normal(n_ray, dtype=np.float32)).to(dev)
-        aty = back(y)
-        lhs = float(torch.dot(sino[0].double(), y.double()))
-        rhs = float(torch.dot(vol.reshape(-1).double(), aty.reshape(-1).double()))
+        # Config check: <A x, y> vs <x, A^T y> with y ~ N(0, 1) (seed 2, the timed back
+        # projection's input), FP64 dot products.
+        lhs = float(torch.dot(sino[0].double(), y_dense.double()))
+        rhs = float(torch.dot(vol.reshape(-1).double(), bk.reshape(-1).double()))
         adjoint = abs(lhs - rhs) / abs(lhs)
-        del y, aty
+
+    # Beside the headline: the back projection of the forward projection itself,
+    # whose zero rays the back projectors skip (not the timed workload).
+    sparse_ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    sparse_ev[0].record()
+    back(sino)
+    sparse_ev[1].record()
+    torch.cuda.synchronize()
+    back_ms_sparse = sparse_ev[0].elapsed_time(sparse_ev[1])
+    zero_rays_sparse = int((sino == 0).sum().item())
 
     fwd_bytes = 4 * S + 4 * n_ray + 24 * n_ray
     back_bytes = 4 * S + 4 * n_ray + 24 * n_ray
@@ -561,11 +576,15 @@ def main(argv=None) -> int:
             "semantics": a.mode,
             "schedule": "plain" if a.plain_order else "detector-tiled (64 rows x 4 cols per block)",
             "back_projector": a.back,
+            "back_input": "dense y ~ N(0,1), seed 2 (rays skipped for a zero value: %d of %d)" % (n_zero, n_ray),
+            "segments_traced_back": S,
             "adjoint_rel_err": adjoint,
             "fwd_ms": fwd_ms,
             "back_ms": back_ms,
             "voxel_updates_per_s": 2 * S * a.steps / total_s,
             "pair_hbm_frac": pair_bytes / (1e-3 * (fwd_ms + back_ms)) / PEAK_HBM,
+            "back_ms_of_forward_output": back_ms_sparse,
+            "zero_rays_of_forward_output": zero_rays_sparse,
         },
         "roofline": {
             "kernel": dom,

@@ -16,12 +16,16 @@ and carry a leading batch dimension.
 The 3D on-the-fly functions take rays without a detector layout, as the
 reference's call sites pass them.  When the rays are a stack of flat-panel
 views ([view][rows][cols], one source per view: the cone-beam and helical
-builders' order), the layout is recognised once per ray set (cached, keyed by
-the tensors' identity and version counters) and used: the forward gets the
-detector-tiled schedule (same results) and the back projection runs on LDS
-bricks instead of one global atomic per segment (same segments, fixed-point
-sums).  Recognising a layout reads the rays and synchronises once; a brick
-plan is cached per (ray set, grid, M_inv and b values).
+builders' order), the layout is recognised once per ray set and used: the
+forward gets the detector-tiled schedule (same results) and the back
+projection runs on LDS bricks instead of one global atomic per segment (same
+segments, fixed-point sums).  Recognising a layout reads the rays and
+synchronises once; a brick plan is cached per (ray set, grid, M_inv and b
+values).  Cache entries hold weak references to the ray tensors and are used
+only while those references return the very tensors of the call, unwritten
+(version counters) and at the same storage address, so a new ray set that the
+caching allocator places in a freed block never meets an old entry.  The
+values of M_inv and b are read once per tensor object and version.
 
 Likewise a 2D crossing table that compute_intersections_2d returned, passed
 back untouched with the same rays, grid and map, is projected on the fly with
@@ -577,8 +581,12 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
 _CACHE_SIZE = 4
 _MIN_VIEW_RAYS = 64
 _MIN_VALID_VIEWS = 0.9
-_LAYOUTS: OrderedDict = OrderedDict()  # ray identity -> (rows, cols) or None
-_PLANS: OrderedDict = OrderedDict()    # (ray identity, layout, grid, M_inv and b values) -> plan or None
+# ray key -> (src ref, dst ref, (rows, cols) or None)
+_LAYOUTS: OrderedDict = OrderedDict()
+# (ray key, layout, grid, M_inv and b values) -> (src ref, dst ref, plan or None)
+_PLANS: OrderedDict = OrderedDict()
+# id(map tensor) -> (weak reference, storage address, version, values) of M_inv / b tensors read once
+_MAP_VALUES: dict = {}
 _CACHE_LOCK = threading.RLock()        # the caches are shared by the host threads of a process
 
 
@@ -590,8 +598,61 @@ def _remember(cache: OrderedDict, key, value):
     return value
 
 
-def _ray_identity(src, dst):
+def _ray_key(src, dst):
+    """Key of a ray set: device, storage addresses, count and version counters.
+    An entry under this key is only used while weak references taken with it
+    still return these very tensors: the caching allocator hands a freed block
+    to the next tensor of that size, so address and version alone could name a
+    new ray set."""
     return (str(src.device), src.data_ptr(), dst.data_ptr(), int(src.shape[0]), src._version, dst._version)
+
+
+def _alive(ref, t) -> bool:
+    return ref() is t
+
+
+def _map_values(t):
+    """Values of a small map tensor (M_inv or b) as a tuple.  Read from the
+    device once per tensor object, storage and version; later calls with the
+    same unwritten tensor do not synchronise."""
+    key = id(t)
+    with _CACHE_LOCK:
+        e = _MAP_VALUES.get(key)
+        if e is not None and e[0]() is t and e[1] == t.data_ptr() and e[2] == t._version:
+            return e[3]
+    vals = tuple(t.detach().flatten().tolist())
+    with _CACHE_LOCK:
+        if len(_MAP_VALUES) > 64:
+            for k in [k for k, v in _MAP_VALUES.items() if v[0]() is None]:
+                del _MAP_VALUES[k]
+        _MAP_VALUES[key] = (weakref.ref(t), t.data_ptr(), t._version, vals)
+    return vals
+
+
+# id(M) -> (weak reference, storage address, version, inverse) of affine matrices inverted here
+_INVERSES: dict = {}
+
+
+def cached_inverse(M):
+    """float32 contiguous inverse of a square map M, computed once per tensor
+    object, storage and version.  torch.inverse checks its result on the host
+    (a synchronisation) and returns a new tensor each time; reusing one inverse
+    object also lets the per-tensor value cache above answer without reading
+    the device.  Under graph capture nothing is cached."""
+    if _capturing():
+        return torch.inverse(M.to(torch.float32)).to(torch.float32).contiguous()
+    key = id(M)
+    with _CACHE_LOCK:
+        e = _INVERSES.get(key)
+        if e is not None and e[0]() is M and e[1] == M.data_ptr() and e[2] == M._version:
+            return e[3]
+    inv = torch.inverse(M.to(torch.float32)).to(torch.float32).contiguous()
+    with _CACHE_LOCK:
+        if len(_INVERSES) > 64:
+            for k in [k for k, v in _INVERSES.items() if v[0]() is None]:
+                del _INVERSES[k]
+        _INVERSES[key] = (weakref.ref(M), M.data_ptr(), M._version, inv)
+    return inv
 
 
 def detect_layout(src, dst):
@@ -633,27 +694,36 @@ def _layout_of(src, dst):
     would synchronise), so captured work never depends on the cache."""
     if not AUTO_LAYOUT or _capturing():
         return None
-    key = _ray_identity(src, dst)
+    key = _ray_key(src, dst)
     with _CACHE_LOCK:
-        if key in _LAYOUTS:
+        e = _LAYOUTS.get(key)
+        if e is not None and _alive(e[0], src) and _alive(e[1], dst):
             _LAYOUTS.move_to_end(key)
-            return _LAYOUTS[key]
-        return _remember(_LAYOUTS, key, detect_layout(src, dst))
+            return e[2]
+    layout = detect_layout(src, dst)
+    with _CACHE_LOCK:
+        _remember(_LAYOUTS, key, (weakref.ref(src), weakref.ref(dst), layout))
+    return layout
 
 
 def _plan_of(src, dst, layout, M_inv, b, shape):
     """Brick plan for the recognised layout, or None when fewer than 90% of the
-    views fit the flat-panel model (the brick would then read whole detectors)."""
-    key = (_ray_identity(src, dst), layout, shape, tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()))
+    views fit the flat-panel model (the brick would then read whole detectors).
+    Cached per ray set, layout, grid and map values; a repeated call with the
+    same tensors reads nothing back from the device."""
+    key = (_ray_key(src, dst), layout, shape, _map_values(M_inv), _map_values(b))
     with _CACHE_LOCK:
-        if key in _PLANS:
+        e = _PLANS.get(key)
+        if e is not None and _alive(e[0], src) and _alive(e[1], dst):
             _PLANS.move_to_end(key)
-            return _PLANS[key]
-        frames = cone_frames(src, dst, layout, M_inv, b, shape)
-        plan = None
-        if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
-            plan = brick_plan(frames, layout, shape)
-        return _remember(_PLANS, key, plan)
+            return e[2]
+    frames = cone_frames(src, dst, layout, M_inv, b, shape)
+    plan = None
+    if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
+        plan = brick_plan(frames, layout, shape)
+    with _CACHE_LOCK:
+        _remember(_PLANS, key, (weakref.ref(src), weakref.ref(dst), plan))
+    return plan
 
 
 # ---------------------------------------------------------------------------
@@ -673,7 +743,7 @@ _TABLE2D_COMPARE_BYTES = 256 << 20  # table bytes computed per comparison chunk
 
 def _geometry2d(n_row, n_col, src, dst, M_inv, b):
     return (int(n_row), int(n_col), weakref.ref(src), src._version, weakref.ref(dst), dst._version,
-            tuple(M_inv.flatten().tolist()), tuple(b.flatten().tolist()))
+            _map_values(M_inv), _map_values(b))
 
 
 def _prune(cache):
@@ -710,8 +780,7 @@ def _table2d_rule(tvals, src, dst, M_inv, b, n_row, n_col):
     if e is not None and e[0]() is tvals and tvals._version == e[1]:
         nr, nc, sref, sver, dref, dver, mv, bv = e[2]
         if ((nr, nc) == (int(n_row), int(n_col)) and sref() is src and dref() is dst and src._version == sver and
-                dst._version == dver and tuple(M_inv.flatten().tolist()) == mv and
-                tuple(b.flatten().tolist()) == bv):
+                dst._version == dver and _map_values(M_inv) == mv and _map_values(b) == bv):
             return e[3]
     return _compare_table2d(tvals, src, dst, M_inv, b, n_row, n_col)
 

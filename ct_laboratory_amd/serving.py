@@ -104,6 +104,8 @@ class _Active:
     def __init__(self, req, stream, device):
         self.req = req
         self.stream = stream
+        # the request's measurements were written on the caller's stream
+        stream.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(stream):
             self.proj = _projector(req, device)
             self.map = QuadraticMAP(self.proj, req.y, req.var, req.beta, torch.zeros_like(req.x_true))
@@ -145,7 +147,11 @@ class ReconServer:
             for rid in list(active):
                 a, slot, need = active[rid]
                 if a.done.query():
-                    results[rid] = (a.map.estimate.clone(), time.perf_counter() - t0 - a.req.arrival)
+                    # the copy is queued on the slot's stream, so the next request admitted to
+                    # this slot (whose buffers may reuse this request's blocks) runs after it
+                    with torch.cuda.stream(a.stream):
+                        est = a.map.estimate.clone()
+                    results[rid] = (est, time.perf_counter() - t0 - a.req.arrival)
                     free.append(slot)
                     used -= need
                     del active[rid]

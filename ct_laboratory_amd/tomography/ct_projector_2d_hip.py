@@ -32,7 +32,7 @@ def _M_inv(M: torch.Tensor) -> torch.Tensor:
     """Contiguous float32 inverse on M's device (ct_projector_2d_cuda.py:6-8)."""
     if M.dim() > 2:
         M = M[0]
-    return torch.inverse(M.to(torch.float32)).to(torch.float32).contiguous()
+    return _C.cached_inverse(M)
 
 
 def _b(b: torch.Tensor) -> torch.Tensor:

@@ -63,7 +63,7 @@ def _prepare_M_inv(M: torch.Tensor) -> torch.Tensor:
     """Row-major float32 inverse of M on M's device (ct_projector_3d_cuda.py:42-48)."""
     if not M.is_cuda:
         raise TypeError("M must be a HIP (cuda) tensor")
-    return torch.inverse(M.to(torch.float32)).to(torch.float32).contiguous()
+    return _C.cached_inverse(M)
 
 
 def _f32c(t: torch.Tensor) -> torch.Tensor:

@@ -352,3 +352,34 @@ def test_auto_layout_uses_bricks(dev, mname, mode):
     moved = _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
     ref2 = O.back(y.cpu().numpy(), shape, np.eye(3, dtype=np.float32), b, ts.cpu().numpy(), dst.numpy(), mode=mname)
     assert rel_err(moved.cpu().numpy(), ref2) < 1e-5
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_auto_layout_new_rays_in_freed_block(dev, mname, mode):
+    """A ray set freed after its layout and brick plan were cached, followed by a
+    new ray set of the same size (another angle offset) that the caching
+    allocator may place in the same blocks with the same version counters: the
+    new set gets its own plan, and its back projection matches the oracle.  A
+    second call with the same map tensors reads no map values from the device."""
+    shape = (40, 36, 28)
+    b = np.array([-(n - 1) / 2 for n in shape], np.float32)
+    tm, tb = _t(np.eye(3), dev), _t(b, dev)
+    rng = np.random.default_rng(23)
+    src_a, dst_a = build_conebeam_3d_geometry(8, 40, 24, 600.0, 900.0, det_spacing=2.0)
+    ts, td = src_a.to(dev).contiguous(), dst_a.to(dev).contiguous()
+    y = _t(rng.uniform(0, 1, (1, src_a.shape[0])), dev)
+    _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
+    assert _C._layout_of(ts, td) == (24, 40)
+    ptrs_a = (ts.data_ptr(), td.data_ptr())
+    del ts, td
+    src_b, dst_b = build_conebeam_3d_geometry(8, 40, 24, 600.0, 900.0, det_spacing=2.0, angle_offset=0.37)
+    ts, td = src_b.to(dev).contiguous(), dst_b.to(dev).contiguous()
+    print("same blocks as the freed ray set:", (ts.data_ptr(), td.data_ptr()) == ptrs_a)
+    got = _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
+    ref = O.back(y.cpu().numpy(), shape, np.eye(3, dtype=np.float32), b, src_b.numpy(), dst_b.numpy(), mode=mname)
+    assert rel_err(got.cpu().numpy(), ref) < 1e-5
+    # map values come from the per-tensor cache on a repeated call
+    key_m = _C._MAP_VALUES.get(id(tm))
+    assert key_m is not None and key_m[0]() is tm
+    again = _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
+    assert torch.equal(again, got)
