@@ -422,6 +422,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-views", type=int, default=8)
     ap.add_argument("--plain-order", action="store_true", help="one ray per lane in storage order (no tiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the full-size adjoint check")
+    ap.add_argument("--no-sparse", action="store_true",
+                    help="skip the extra back projection of the forward output (profiling runs)")
     ap.add_argument("--back", choices=["brick", "atomic"], default="brick",
                     help="back projector: LDS bricks (no global atomics) or per-ray atomics")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -541,13 +543,15 @@ def main(argv=None) -> int:
 
     # Beside the headline: the back projection of the forward projection itself,
     # whose zero rays the back projectors skip (not the timed workload).
-    sparse_ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    sparse_ev[0].record()
-    back(sino)
-    sparse_ev[1].record()
-    torch.cuda.synchronize()
-    back_ms_sparse = sparse_ev[0].elapsed_time(sparse_ev[1])
-    zero_rays_sparse = int((sino == 0).sum().item())
+    back_ms_sparse = zero_rays_sparse = None
+    if not a.no_sparse:
+        sparse_ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        sparse_ev[0].record()
+        back(sino)
+        sparse_ev[1].record()
+        torch.cuda.synchronize()
+        back_ms_sparse = sparse_ev[0].elapsed_time(sparse_ev[1])
+        zero_rays_sparse = int((sino == 0).sum().item())
 
     fwd_bytes = 4 * S + 4 * n_ray + 24 * n_ray
     back_bytes = 4 * S + 4 * n_ray + 24 * n_ray
