@@ -70,6 +70,8 @@ _SIGS = {
     "ctlab_brick_plan_3d": [_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P],
     "ctlab_back_project_3d_brick": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
                                     _P, _P, _P],
+    "ctlab_back_project_3d_brick_noalloc": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _INT,
+                                            _INT, _P, _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }

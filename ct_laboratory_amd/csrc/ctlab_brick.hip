@@ -27,6 +27,8 @@
 // used to bound the shadow, conservatively.  A view whose rays do not fit the
 // flat-panel model within a quarter pixel (source moving, detector not
 // affine) is marked invalid and uses its whole detector for every brick.
+#include <mutex>
+
 #include "ctlab_common.h"
 #include "fixed_point.h"
 #include "siddon_traverse.h"
@@ -767,12 +769,14 @@ extern "C" int ctlab_brick_plan_3d(const float* frames, int64_t n_view, int64_t 
     return check_launch(fn);
 }
 
-extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
-                                           int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
-                                           const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
-                                           int64_t n_z, int mode, int accumulate, float* out, float* view_max,
-                                           ctlab_stream_t stream) {
-    const char* fn = "back_project_3d_brick";
+namespace ctlab {
+
+// Shared by both brick entry points: checks, per-view maxima into view_max
+// (batch * n_view floats, every entry written), the zero fill and the launch.
+static int back_brick(const char* fn, const float* sino, int64_t batch, const float* src, const float* dst,
+                      int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan, const float* M_inv,
+                      const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
+                      float* view_max, ctlab_stream_t stream) {
     const int64_t n[3] = {n_x, n_y, n_z};
     int64_t V = 1;
     for (int a = 0; a < 3; ++a) {
@@ -817,4 +821,60 @@ extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, con
                            n_ray, V, d, bg, (int)n_view, (int)det_rows, (int)det_cols, plan, totals, view_e, frames, src,
                            dst, M_inv, b, view_max, accumulate, split);
     return check_launch(fn);
+}
+
+// Per-view maxima of the entry point that allocates nothing.  In host code the
+// name `scratch` is only a stand-in: kernels and copies get the device address
+// that hipGetSymbolAddress returns for the current device, kept per device.
+static __device__ __align__(256) float scratch[CTLAB_BRICK_SCRATCH_FLOATS];
+
+constexpr int kMaxDevices = 64;
+static float* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_lock;
+
+// Device address of this device's copy of `scratch` (looked up once per device,
+// with that device current).
+static int scratch_of_current_device(const char* fn, float** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(CTLAB_EHIP, "%s: hipGetDevice: %s", fn, hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CTLAB_EINVAL, "%s: device %d beyond %d", fn, dev, kMaxDevices);
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CTLAB_EHIP, "%s: hipGetSymbolAddress(scratch): %s", fn, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CTLAB_OK;
+}
+
+}  // namespace ctlab
+
+extern "C" int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* src, const float* dst,
+                                           int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
+                                           const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
+                                           int64_t n_z, int mode, int accumulate, float* out, float* view_max,
+                                           ctlab_stream_t stream) {
+    return ctlab::back_brick("back_project_3d_brick", sino, batch, src, dst, n_view, det_rows, det_cols, plan, M_inv,
+                             b, n_x, n_y, n_z, mode, accumulate, out, view_max, stream);
+}
+
+extern "C" int ctlab_back_project_3d_brick_noalloc(const float* sino, int64_t batch, const float* src,
+                                                   const float* dst, int64_t n_view, int64_t det_rows,
+                                                   int64_t det_cols, const int32_t* plan, const float* M_inv,
+                                                   const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
+                                                   int accumulate, float* out, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "back_project_3d_brick_noalloc";
+    CTLAB_REQUIRE(batch >= 1 && n_view >= 1 && batch <= CTLAB_BRICK_SCRATCH_FLOATS / n_view,
+                  "%s: batch * n_view = %lld * %lld exceeds the %d per-view maxima of the static workspace", fn,
+                  (long long)batch, (long long)n_view, (int)CTLAB_BRICK_SCRATCH_FLOATS);
+    float* view_max = nullptr;
+    const int st = scratch_of_current_device(fn, &view_max);
+    if (st) return st;
+    return back_brick(fn, sino, batch, src, dst, n_view, det_rows, det_cols, plan, M_inv, b, n_x, n_y, n_z, mode,
+                      accumulate, out, view_max, stream);
 }

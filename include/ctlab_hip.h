@@ -248,6 +248,27 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
                                 int accumulate, float* out, float* view_max, ctlab_stream_t stream);
 
+/* Largest batch * n_view of ctlab_back_project_3d_brick_noalloc (256 KiB of
+ * per-view maxima, held by each device the library runs on until the process
+ * ends). */
+#define CTLAB_BRICK_SCRATCH_FLOATS 65536
+
+/* ctlab_back_project_3d_brick without the view_max workspace: the per-view
+ * maxima go to a static device array of CTLAB_BRICK_SCRATCH_FLOATS floats, so
+ * the call allocates nothing (no hipMalloc, hipFree or allocating library
+ * routine).  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments and results as
+ * ctlab_back_project_3d_brick otherwise; batch * n_view above
+ * CTLAB_BRICK_SCRATCH_FLOATS is rejected with CTLAB_EINVAL.  Every call on one
+ * device shares that device's array (each call writes all the entries it
+ * reads), so calls that overlap on one device (different streams, or
+ * concurrent host threads) race; serialise them. */
+int ctlab_back_project_3d_brick_noalloc(const float* sino, int64_t batch, const float* src, const float* dst,
+                                        int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
+                                        const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
+                                        int mode, int accumulate, float* out, ctlab_stream_t stream);
+
 /* ---------------------------------------------------------------------- */
 /* Many 2D scans in one persistent launch (no reference counterpart: the   */
 /* reference projects scans one launch each, ct_projector_2d_cuda.py:23-46) */
