@@ -163,9 +163,10 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
 # ---------------------------------------------------------------------------
 
 
-def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N.MODE_OTF,
+def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N.MODE_MARCH,
                                        zero_rule: int = N.ZERO_THRESH, out=None):
-    """[B, n_ray] line integrals (src/ct_projector_2d.cu:416-440)."""
+    """[B, n_ray] line integrals (src/ct_projector_2d.cu:416-440).  The default
+    mode is the reference kernel's march (src/ct_projector_2d.cu:246-331)."""
     name = "forward_project_2d_on_the_fly_cuda"
     _check_f32(name, img, src, dst, M_inv, b)
     batch, (nr, nc) = _vol_dims(name, img, 2)
@@ -178,9 +179,12 @@ def forward_project_2d_on_the_fly_cuda(img, src, dst, M_inv, b, *, mode: int = N
     return out
 
 
-def back_project_2d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_row, n_col, *, mode: int = N.MODE_OTF,
+def back_project_2d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_row, n_col, *, mode: int = N.MODE_MARCH,
                                     zero_rule: int = N.ZERO_THRESH, out=None):
-    """[B, n_row, n_col] back projection (src/ct_projector_2d.cu:442-465)."""
+    """[B, n_row, n_col] back projection (src/ct_projector_2d.cu:442-465).  The
+    default mode is the reference kernel's march (src/ct_projector_2d.cu:334-413),
+    whose back projection adds each step with a float atomic; the other modes
+    run on LDS tiles."""
     name = "back_project_2d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)

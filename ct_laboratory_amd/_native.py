@@ -23,10 +23,11 @@ _INT = ctypes.c_int
 
 MODE_TORCH = 0
 MODE_OTF = 1
+MODE_MARCH = 2
 ZERO_THRESH = 0
 ZERO_EXACT = 1
 
-MODES = {"torch": MODE_TORCH, "otf": MODE_OTF}
+MODES = {"torch": MODE_TORCH, "otf": MODE_OTF, "march": MODE_MARCH}
 COMPRESS_CUDA = 0
 COMPRESS_TORCH = 1
 

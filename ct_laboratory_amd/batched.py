@@ -60,6 +60,8 @@ class Scan2D:
         if self.src.dtype != torch.float32 or self.src.shape != self.dst.shape or self.src.shape[1] != 2:
             raise ValueError("src/dst must be float32 [n_ray, 2] of equal shape")
         self.n_ray = int(self.src.shape[0])
+        if semantics not in ("torch", "otf"):
+            raise ValueError(f"the multi-scan launches take the 'torch' or 'otf' rule, not {semantics!r}")
         self.mode = semantics_mode(semantics)
         self.zero_rule = N.ZERO_EXACT if semantics == "torch" else N.ZERO_THRESH
         self.geometry_class = geometry_class

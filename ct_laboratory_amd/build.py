@@ -23,6 +23,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
@@ -63,13 +64,28 @@ def lib_path() -> str:
 
 
 def build(verbose: bool = False, isa: bool = False) -> str:
+    """Compiles every source to an object in parallel (one hipcc per source),
+    then links the shared library."""
     os.makedirs(OUT_DIR, exist_ok=True)
+    obj_dir = os.path.join(OUT_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
     out = lib_path()
     tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + sources()
+    objs = [os.path.join(obj_dir, os.path.basename(src).replace(".hip", ".o")) for src in sources()]
+    cmds = [[hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, src] for src, obj in zip(sources(), objs)]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        for c in cmds:
+            print(" ".join(c), flush=True)
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        for r in pool.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                raise subprocess.CalledProcessError(r.returncode, r.args)
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(tmp, out)
     if isa:
         dump_isa(verbose=verbose)

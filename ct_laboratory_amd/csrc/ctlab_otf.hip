@@ -4,7 +4,8 @@
 // One lane traces one ray (siddon_traverse.h).  The forward kernel gathers
 // volume values along the ray and writes one line integral per lane; the back
 // kernel scatters sino*length with float atomic adds (global_atomic_add_f32,
-// no return value).  In the forward, blockIdx.y is a group of up to NB batch
+// no return value).  2D rays may also take the reference's 2D on-the-fly march
+// (MODE_MARCH, march_ray_2d), whose back projection always uses the atomics.  In the forward, blockIdx.y is a group of up to NB batch
 // members: a lane traces its ray once and gathers from each member's volume,
 // so the traversal arithmetic (which bounds the kernel) is shared by the
 // group.  In the back projection blockIdx.y is the batch index.
@@ -118,7 +119,9 @@ __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict_
         }
         pl = len;
     };
-    if (g.diag)
+    if constexpr (MODE == MODE_MARCH)
+        march_ray_2d(g, s, d, gather);
+    else if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, gather);
     else
         trace_ray<ND, MODE, false>(g, s, d, gather);
@@ -164,7 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_back_otf(const float* __restrict__ s
     load_ray<ND>(src, dst, r, s, d);
     float* __restrict__ o = out + bi * vol_elems;
     auto scatter = [&](uint32_t idx, float len) { unsafeAtomicAdd(o + idx, val * len); };
-    if (g.diag)
+    if constexpr (MODE == MODE_MARCH)
+        march_ray_2d(g, s, d, scatter);
+    else if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, scatter);
     else
         trace_ray<ND, MODE, false>(g, s, d, scatter);
@@ -191,7 +196,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(int64_t n_ray, Dims<ND> dims, 
         }
         ++k;
     };
-    if (g.diag)
+    if constexpr (MODE == MODE_MARCH)
+        march_ray_2d(g, s, d, record);
+    else if (g.diag)
         trace_ray<ND, MODE, true>(g, s, d, record);
     else
         trace_ray<ND, MODE, false>(g, s, d, record);
@@ -217,7 +224,8 @@ static int check_common(const char* fn, const int64_t* n, int64_t batch, int64_t
     CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range [1, 65535]", fn, (long long)batch);
     CTLAB_REQUIRE(n_ray >= 0, "%s: n_ray %lld < 0", fn, (long long)n_ray);
     CTLAB_REQUIRE(grid_for(n_ray) <= kMaxGridX, "%s: too many rays", fn);
-    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF || (ND == 2 && mode == CTLAB_MODE_MARCH),
+                  "%s: unknown mode %d", fn, mode);
     if (n_ray > 0) {
         CTLAB_REQUIRE(src && dst, "%s: src/dst is null", fn);
         CTLAB_REQUIRE(minv && b, "%s: M_inv/b is null", fn);
@@ -269,6 +277,13 @@ static int forward_otf(const char* fn, const float* vol, int64_t batch, const in
     CTLAB_REQUIRE(vol && out, "%s: volume/out is null", fn);
     int64_t V = 1;
     for (int a = 0; a < ND; ++a) V *= n[a];
+    if constexpr (ND == 2) {
+        if (mode == CTLAB_MODE_MARCH) {
+            launch_forward_otf<2, MODE_MARCH>(vol, out, sched, V, make_dims<2>(n), src, dst, minv, b, zero_rule,
+                                              batch, stream);
+            return check_launch(fn);
+        }
+    }
     if (mode == CTLAB_MODE_TORCH)
         launch_forward_otf<ND, MODE_TORCH>(vol, out, sched, V, make_dims<ND>(n), src, dst, minv, b, zero_rule, batch,
                                            stream);
@@ -288,8 +303,8 @@ static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_
     st = make_sched(fn, n_ray, rows, cols, &sched);
     if (st) return st;
     CTLAB_REQUIRE(out, "%s: out is null", fn);
-    if constexpr (ND == 2) {  // atomic-free: LDS tiles (ctlab_tile2d.hip)
-        if (n_ray > 0) {
+    if constexpr (ND == 2) {  // atomic-free: LDS tiles (ctlab_tile2d.hip); the march rule takes atomics
+        if (n_ray > 0 && mode != CTLAB_MODE_MARCH) {
             CTLAB_REQUIRE(sino, "%s: sino is null", fn);
             return back_project_2d_tiles(fn, sino, batch, n_ray, src, dst, minv, b, n, mode, zero_rule, accumulate,
                                          out, stream);
@@ -304,6 +319,13 @@ static int back_otf(const char* fn, const float* sino, int64_t batch, int64_t n_
     if (n_ray == 0) return CTLAB_OK;
     CTLAB_REQUIRE(sino, "%s: sino is null", fn);
     dim3 grid(sched.n_blocks, (unsigned)batch);
+    if constexpr (ND == 2) {
+        if (mode == CTLAB_MODE_MARCH) {
+            hipLaunchKernelGGL((k_back_otf<2, MODE_MARCH>), grid, dim3(kBlock), 0, stream, sino, out, sched, V,
+                               make_dims<2>(n), src, dst, minv, b, zero_rule);
+            return check_launch(fn);
+        }
+    }
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL((k_back_otf<ND, MODE_TORCH>), grid, dim3(kBlock), 0, stream, sino, out, sched, V,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule);
@@ -322,6 +344,13 @@ static int trace(const char* fn, const int64_t* n, const float* src, const float
     CTLAB_REQUIRE(cap >= 0, "%s: cap < 0", fn);
     if (n_ray == 0) return CTLAB_OK;
     CTLAB_REQUIRE(counts && (cap == 0 || (idx && len)), "%s: output is null", fn);
+    if constexpr (ND == 2) {
+        if (mode == CTLAB_MODE_MARCH) {
+            hipLaunchKernelGGL((k_trace<2, MODE_MARCH>), dim3(grid_for(n_ray)), dim3(kBlock), 0, stream, n_ray,
+                               make_dims<2>(n), src, dst, minv, b, zero_rule, cap, idx, len, counts);
+            return check_launch(fn);
+        }
+    }
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL((k_trace<ND, MODE_TORCH>), dim3(grid_for(n_ray)), dim3(kBlock), 0, stream, n_ray,
                            make_dims<ND>(n), src, dst, minv, b, zero_rule, cap, idx, len, counts);

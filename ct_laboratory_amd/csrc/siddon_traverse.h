@@ -35,7 +35,7 @@
 
 namespace ctlab {
 
-enum : int { MODE_TORCH = 0, MODE_OTF = 1 };
+enum : int { MODE_TORCH = 0, MODE_OTF = 1, MODE_MARCH = 2 };
 enum : int { ZERO_THRESH = 0, ZERO_EXACT = 1 };
 
 // Uniform per-launch geometry.  minv is row-major ND x ND.
@@ -673,6 +673,97 @@ __host__ __device__ __forceinline__ int emit_crossings(const Grid<ND>& g, const 
         ++count;
     }
     return count;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's 2D on-the-fly kernels (MODE_MARCH)
+// ---------------------------------------------------------------------------
+
+// Float to int the way the GPU's conversion instructions do it (NaN gives 0,
+// large magnitudes saturate), except that the saturation is at +-2^30 rather
+// than the int limits: such an index is far outside any grid either way, and
+// stays outside (and free of overflow) over the march's bounded step count.
+// (A plain C cast is undefined for these inputs.)
+__host__ __device__ __forceinline__ int cvt_sat_i32(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 1073741824.0f) return 1073741824;
+    if (x <= -1073741824.0f) return -1073741824;
+    return (int)x;
+}
+
+// |(vx, vy)|: the square root of the double sum of squares, rounded to float.
+// Host and device compute it with the same correctly rounded double operations.
+__host__ __device__ __forceinline__ float hypot_f32(float vx, float vy) {
+    const double s = __builtin_fma((double)vx, (double)vx, (double)vy * (double)vy);
+    return (float)__builtin_sqrt(s);
+}
+
+// Visits the steps of the incremental Siddon march of
+// forward/back_project_2d_on_the_fly_kernel (src/ct_projector_2d.cu:246-331,
+// 334-413), in order, as visit(flat_index, seg) for every step whose (row,
+// column) lies in the grid, including steps of zero or negative length: the
+// entry voxel is taken by floor (ceil - 1 going down) of the entry point's
+// coordinate, while voxel r spans [r - 1/2, r + 1/2], so an entry coordinate
+// with a fractional part of 1/2 or more starts one voxel early; the first
+// crossing then lies behind the entry point, and its step has negative length
+// (t accumulated: t_next += |1/v|, seg = (t_hit - t_cur) * |dst - src|).
+// Every operation follows the CUDA source as nvcc compiles it by default: the
+// forms a*b + c are fused (the affine map, the entry point, the first
+// crossing), the rest rounded separately.  The ray length is hypot_f32 (the
+// reference calls CUDA's hypotf, a different rounding of the same quantity).
+// The step count is capped at 2 (n_row + n_col) + 8, which finite rays never
+// reach (each step moves one index by one across the box, plus the entry
+// correction); the cap only bounds the loop for non-finite input.
+template <class Visit>
+__host__ __device__ __forceinline__ void march_ray_2d(const Grid<2>& g, const float* src, const float* dst,
+                                                      Visit&& visit) {
+    const float vx = dst[0] - src[0], vy = dst[1] - src[1];
+    const float ray_len = hypot_f32(vx, vy);
+    if (ray_len < 1e-12f) return;
+    float sv[2], dv[2];
+    to_voxel<2, MODE_OTF>(g, src, sv);
+    to_voxel<2, MODE_OTF>(g, dst, dv);
+    const float sr = sv[0], sc = sv[1];
+    const float vr = dv[0] - sr, vc = dv[1] - sc;
+    const float inv_vr = (fabsf(vr) < 1e-12f) ? 1e32f : 1.0f / vr;
+    const float inv_vc = (fabsf(vc) < 1e-12f) ? 1e32f : 1.0f / vc;
+    const int n_row = g.n[0], n_col = g.n[1];
+    const float t_r0 = (-0.5f - sr) * inv_vr;
+    const float t_r1 = (((float)n_row - 0.5f) - sr) * inv_vr;
+    const float t_c0 = (-0.5f - sc) * inv_vc;
+    const float t_c1 = (((float)n_col - 0.5f) - sc) * inv_vc;
+    float t_enter = fmaxf(fminf(t_r0, t_r1), fminf(t_c0, t_c1));
+    float t_exit = fminf(fmaxf(t_r0, t_r1), fmaxf(t_c0, t_c1));
+    if (t_exit <= t_enter) return;
+    t_enter = fmaxf(t_enter, 0.0f);
+    t_exit = fminf(t_exit, 1.0f);
+    const float r_enter = __builtin_fmaf(t_enter, vr, sr);
+    const float c_enter = __builtin_fmaf(t_enter, vc, sc);
+    const bool up_r = vr >= 0.0f, up_c = vc >= 0.0f;
+    int ri = up_r ? cvt_sat_i32(floorf(r_enter)) : cvt_sat_i32(ceilf(r_enter)) - 1;
+    int ci = up_c ? cvt_sat_i32(floorf(c_enter)) : cvt_sat_i32(ceilf(c_enter)) - 1;
+    float t_next_r = up_r ? __builtin_fmaf(((float)ri + 0.5f) - r_enter, inv_vr, t_enter)
+                          : __builtin_fmaf(((float)ri - 0.5f) - r_enter, inv_vr, t_enter);
+    float t_next_c = up_c ? __builtin_fmaf(((float)ci + 0.5f) - c_enter, inv_vc, t_enter)
+                          : __builtin_fmaf(((float)ci - 0.5f) - c_enter, inv_vc, t_enter);
+    const float dt_r = fabsf(inv_vr), dt_c = fabsf(inv_vc);
+    const int step_r = up_r ? 1 : -1, step_c = up_c ? 1 : -1;
+    float t_cur = t_enter;
+    const int cap = 2 * (n_row + n_col) + 8;
+    for (int k = 0; k < cap && t_cur < t_exit; ++k) {
+        const bool step_row = t_next_r < t_next_c;
+        const float t_hit = fminf(step_row ? t_next_r : t_next_c, t_exit);
+        if (ri >= 0 && ri < n_row && ci >= 0 && ci < n_col)
+            visit((uint32_t)ri * (uint32_t)n_col + (uint32_t)ci, (t_hit - t_cur) * ray_len);
+        t_cur = t_hit;
+        if (step_row) {
+            t_next_r += dt_r;
+            ri += step_r;
+        } else {
+            t_next_c += dt_c;
+            ci += step_c;
+        }
+    }
 }
 
 // Segment geometry of one pair of crossings given as t values, in the op order

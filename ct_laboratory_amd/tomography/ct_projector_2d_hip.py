@@ -8,10 +8,14 @@ through ``_C`` (src/ct_projector_2d.cu:246-465, bindings.cpp:75-80).
 The on-the-fly functions take ``semantics``: ``"torch"`` is the segment rule
 of the reference's torch projector (ct_projector_2d_torch.py:68-250: sorted
 crossings with exact ``dir == 0`` tests, world-space lengths, midpoint
-rounded half to even); ``"otf"`` uses (t1-t0)*|dst-src| lengths with 0 and 1
-inserted and the ``|dir| < 1e-12`` test.  The reference's own 2D on-the-fly
-kernel uses floor indexing (src/ct_projector_2d.cu:246-410) and no Python
-code calls it; neither mode reproduces that kernel.
+rounded half to even); ``"otf"`` is the rule of the reference's 3D on-the-fly
+kernel applied to 2D rays ((t1-t0)*|dst-src| lengths with 0 and 1 inserted and
+the ``|dir| < 1e-12`` test); ``"march"`` is the reference's own 2D on-the-fly
+kernel (src/ct_projector_2d.cu:246-413: an incremental march from a
+floor-indexed entry voxel, every in-grid step visited, zero and negative
+lengths included), which ``_C.forward/back_project_2d_on_the_fly_cuda`` use by
+default, as the reference's ``_C`` does.  No reference Python code calls the 2D
+on-the-fly kernels; the default here stays ``"torch"``.
 """
 from __future__ import annotations
 

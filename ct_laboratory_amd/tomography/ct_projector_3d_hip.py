@@ -39,13 +39,14 @@ __all__ = [
     "forward_project_3d_on_the_fly_cuda", "back_project_3d_on_the_fly_cuda",
 ]
 
-_MODES = {"torch": N.MODE_TORCH, "otf": N.MODE_OTF}
+_MODES = {"torch": N.MODE_TORCH, "otf": N.MODE_OTF, "march": N.MODE_MARCH}
 
 
 def semantics_mode(semantics: str) -> int:
-    """Kernel mode of a segment rule name ("torch" or "otf")."""
+    """Kernel mode of a segment rule name ("torch", "otf", or "march": the
+    reference's 2D on-the-fly kernel, 2D rays only)."""
     if semantics not in _MODES:
-        raise ValueError(f"semantics must be 'torch' or 'otf', got {semantics!r}")
+        raise ValueError(f"semantics must be 'torch', 'otf' or 'march', got {semantics!r}")
     return _MODES[semantics]
 
 

@@ -26,6 +26,12 @@
  *                     (src/ct_projector_3d.cu:834-1012): t = (p-0.5-s)*(1/dir),
  *                     endpoints 0 and 1 inserted, length (t1-t0)|dst-src|.
  * In both, a segment's voxel is round-half-even of M_inv (midpoint - b).
+ *   CTLAB_MODE_MARCH  (2D on-the-fly entry points and ctlab_trace_2d only) the
+ *                     incremental march of the reference's 2D on-the-fly kernels
+ *                     (src/ct_projector_2d.cu:246-413): entry voxel by floor of
+ *                     the entry coordinate, t accumulated per step, length
+ *                     (t_hit - t_cur)|dst-src|, every in-grid step visited
+ *                     (zero and negative lengths included).
  */
 #ifndef CTLAB_HIP_H
 #define CTLAB_HIP_H
@@ -43,6 +49,7 @@ extern "C" {
 
 #define CTLAB_MODE_TORCH 0
 #define CTLAB_MODE_OTF 1
+#define CTLAB_MODE_MARCH 2
 
 #define CTLAB_ZERO_THRESH 0 /* |dir| < 1e-12: axis has no crossings (3D torch, all CUDA kernels) */
 #define CTLAB_ZERO_EXACT 1  /* dir == 0: axis has no crossings (ct_projector_2d_torch.py:42) */
@@ -288,7 +295,7 @@ typedef struct ctlab_scan2d {
     float b[2];
     int32_t n_row, n_col;
     int64_t n_ray;
-    int32_t mode;      /* CTLAB_MODE_TORCH or CTLAB_MODE_OTF */
+    int32_t mode;      /* CTLAB_MODE_TORCH or CTLAB_MODE_OTF (any other value takes the torch rule) */
     int32_t zero_rule; /* CTLAB_ZERO_THRESH or CTLAB_ZERO_EXACT */
     int64_t item_begin;
 } ctlab_scan2d;

@@ -10,6 +10,7 @@ returns numpy arrays.  Modes:
 
   "torch"  ct_laboratory/tomography/ct_projector_{2d,3d}_torch.py semantics
   "otf"    src/ct_projector_3d.cu ray_kernel_plane_sort semantics (3d.cu:834-1012)
+  "march"  src/ct_projector_2d.cu 2D on-the-fly march (2d.cu:246-413), 2D only
 
 Parity status of each function is listed in ``PINNED`` below and in the tests
 that pin it (tests/test_oracle.py).
@@ -26,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libsiddon_oracle.so")
 _lib = None
 
-MODES = {"torch": 0, "otf": 1}
+MODES = {"torch": 0, "otf": 1, "march": 2}
 ZERO_RULES = {"thresh": 0, "exact": 1}
 
 # Which check pins each oracle function (tests/test_oracle.py).
@@ -37,13 +38,16 @@ PINNED = {
     "reference examples; exact rational arithmetic on tiny grids; A.1 = chord length; adjoint identity",
     "trace_f64/forward_f64/back_f64": "exact rational arithmetic (index lists equal, lengths to 1e-9); "
     "chord length; adjoint identity to 1e-12",
-    "trace/forward/back otf": "closed forms; A.1 = chord length with the end points inserted; adjoint. "
-    "The exact nvcc contraction of the reference kernel cannot be compiled here: bit patterns of otf "
-    "crossings are parity unpinned",
+    "trace/forward/back otf": "closed forms; A.1 = chord length with the end points inserted; adjoint; "
+    "index lists against exact rational arithmetic with 0 and 1 inserted. The exact nvcc contraction of "
+    "the reference kernel cannot be compiled here: bit patterns of otf crossings are parity unpinned",
     "compress torch, voxel_diag_len": "golden vectors of compress_tvals_to_uint16, bit for bit",
     "forward/back tvals": "golden projections of the reference's own tables",
-    "compress cuda, forward/back u16": "decode error within half a quantum before the 32000 clamp; "
-    "bit patterns of the CUDA rounding (roundf, FMA decode) parity unpinned",
+    "compress cuda, forward/back u16": "known answers for roundf ties, INF entries and rows, the clamp "
+    "(tests/test_oracle_cuda_rules.py); decode = one rounding of start + k*scale against exact rationals; "
+    "u16 projections = table projections of the decoded table; the 1e-9 skip by a known answer",
+    "trace/forward/back march (2D)": "hand-derived known answers of the floor-indexed march, chord length "
+    "where the entry voxel is right, adjoint; the product traversal compiled for the host equals it bit for bit",
 }
 
 
@@ -98,6 +102,8 @@ def _declare(L):
         f = getattr(L, name)
         f.restype = _INT
         f.argtypes = [_P, _P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _I64, _P]
+    L.orc_decode_u16_cuda.restype = _INT
+    L.orc_decode_u16_cuda.argtypes = [_P, _P, _P, _I64, _I64, _P]
     L.orc_num_threads.restype = _INT
     L.orc_set_num_threads.argtypes = [_INT]
 
@@ -285,6 +291,18 @@ def compress_cuda(tvals, src, dst, voxel_diag_len: float):
     lib().orc_compress_cuda(_ptr(tv), N, K, _ptr(src), _ptr(dst), float(voxel_diag_len), _ptr(u), _ptr(st),
                             _ptr(sc))
     return u, st, sc
+
+
+def decode_u16_cuda(u16, start, scale):
+    """The crossings the CUDA compressed consumers decode (3d.cu:126-129):
+    start + k * scale in one fused multiply-add, k the integer running sum."""
+    u = np.ascontiguousarray(np.asarray(u16).view(np.uint16) if np.asarray(u16).dtype == np.int16 else
+                             np.asarray(u16, dtype=np.uint16))
+    st, sc = _f32(start), _f32(scale)
+    N, K = u.shape
+    out = np.zeros((N, K), dtype=np.float32)
+    lib().orc_decode_u16_cuda(_ptr(u), _ptr(st), _ptr(sc), N, K, _ptr(out))
+    return out
 
 
 def compress_torch(tvals, src, dst, voxel_diag_len: float):

@@ -24,6 +24,10 @@
  *               (plane-s)*(1/dir) for |dir|>1e-12, kept if 0<=t<=1, endpoints 0 and 1
  *               inserted, sorted, pairs with t1<=t0 skipped, voxel from the world
  *               point at t_mid, seg = (t1-t0)*|dst-src|.
+ *   MODE_MARCH  src/ct_projector_2d.cu:246-331 and 334-413 (2D on-the-fly kernels):
+ *               slab entry/exit, entry voxel by floor (ceil - 1 going down) of the
+ *               entry coordinate, incremental march with t_next += |1/v|, every
+ *               in-grid step visited with length (t_hit - t_cur)*|dst-src|.
  *   F64         MODE_TORCH evaluated in double precision (adjoint and closed-form
  *               reference).
  *
@@ -41,6 +45,7 @@
 
 #define MODE_TORCH 0
 #define MODE_OTF 1
+#define MODE_MARCH 2
 
 #define ZERO_RULE_THRESH 0 /* |dir| < 1e-12 gives no crossings (3D torch, all CUDA) */
 #define ZERO_RULE_EXACT 1  /* dir == 0 gives no crossings (2D torch, 2d_torch.py:42) */
@@ -165,6 +170,76 @@ static float ray_len_otf(const orc_geom* g, const float* src, const float* dst) 
     return sqrtf(acc);
 }
 
+/* (int) of a float with the GPU conversion's behaviour for NaN (0) and large
+ * magnitudes (saturated; here at +-2^30, out of every grid either way). */
+static int cvt_i32(float x) {
+    if (isnan(x)) return 0;
+    if (x >= 1073741824.0f) return 1073741824;
+    if (x <= -1073741824.0f) return -1073741824;
+    return (int)x;
+}
+
+/* MODE_MARCH (2d.cu:246-331): the steps of the reference's 2D on-the-fly
+ * march.  nvcc contracts a*b + c by default, so those forms are fmaf here: the
+ * affine map (2d.cu:25-28), the entry point (:289-290) and the first crossing
+ * (:295-300); the accumulation t_next += dt is a plain add.  The ray length is
+ * the double sum of squares, square-rooted and rounded to float (the kernel
+ * calls hypotf).  Every step in the grid is written, whatever its length. */
+static int64_t trace_march2d_f32(const orc_geom* g, const float* src, const float* dst, int64_t* idx_out,
+                                 float* len_out) {
+    float vx = dst[0] - src[0], vy = dst[1] - src[1];
+    float ray_len = (float)sqrt(fma((double)vx, (double)vx, (double)vy * (double)vy));
+    if (ray_len < 1e-12f) return 0;
+    float s[2], d[2];
+    to_voxel_mm(g, src, s);
+    to_voxel_mm(g, dst, d);
+    float sr = s[0], sc = s[1];
+    float vr = d[0] - sr, vc = d[1] - sc;
+    float inv_vr = (fabsf(vr) < 1e-12f) ? 1e32f : 1.0f / vr;
+    float inv_vc = (fabsf(vc) < 1e-12f) ? 1e32f : 1.0f / vc;
+    int n_row = (int)g->n[0], n_col = (int)g->n[1];
+    float t_r0 = (-0.5f - sr) * inv_vr;
+    float t_r1 = (((float)n_row - 0.5f) - sr) * inv_vr;
+    float t_c0 = (-0.5f - sc) * inv_vc;
+    float t_c1 = (((float)n_col - 0.5f) - sc) * inv_vc;
+    float t_enter = fmaxf(fminf(t_r0, t_r1), fminf(t_c0, t_c1));
+    float t_exit = fminf(fmaxf(t_r0, t_r1), fmaxf(t_c0, t_c1));
+    if (t_exit <= t_enter) return 0;
+    t_enter = fmaxf(t_enter, 0.0f);
+    t_exit = fminf(t_exit, 1.0f);
+    float r_enter = fmaf(t_enter, vr, sr);
+    float c_enter = fmaf(t_enter, vc, sc);
+    int ri = (vr >= 0.0f) ? cvt_i32(floorf(r_enter)) : cvt_i32(ceilf(r_enter)) - 1;
+    int ci = (vc >= 0.0f) ? cvt_i32(floorf(c_enter)) : cvt_i32(ceilf(c_enter)) - 1;
+    float t_next_r = (vr >= 0.0f) ? fmaf(((float)ri + 0.5f) - r_enter, inv_vr, t_enter)
+                                  : fmaf(((float)ri - 0.5f) - r_enter, inv_vr, t_enter);
+    float t_next_c = (vc >= 0.0f) ? fmaf(((float)ci + 0.5f) - c_enter, inv_vc, t_enter)
+                                  : fmaf(((float)ci - 0.5f) - c_enter, inv_vc, t_enter);
+    float dt_r = fabsf(inv_vr), dt_c = fabsf(inv_vc);
+    float t_cur = t_enter;
+    int64_t nseg = 0;
+    /* bounded like the kernels' loop; finite rays end long before */
+    for (int k = 0; k < 2 * (n_row + n_col) + 8 && t_cur < t_exit; ++k) {
+        int step_row = (t_next_r < t_next_c);
+        float t_hit = step_row ? t_next_r : t_next_c;
+        t_hit = fminf(t_hit, t_exit);
+        if (ri >= 0 && ri < n_row && ci >= 0 && ci < n_col) {
+            idx_out[nseg] = (int64_t)ri * n_col + ci;
+            len_out[nseg] = (t_hit - t_cur) * ray_len;
+            ++nseg;
+        }
+        t_cur = t_hit;
+        if (step_row) {
+            t_next_r += dt_r;
+            ri += (vr >= 0.0f) ? 1 : -1;
+        } else {
+            t_next_c += dt_c;
+            ci += (vc >= 0.0f) ? 1 : -1;
+        }
+    }
+    return nseg;
+}
+
 /*
  * Segments of one ray under a float mode.  Writes every in-range segment
  * (flat index, length), including zero-length ones, in traversal order;
@@ -174,6 +249,7 @@ static int64_t trace_f32(const orc_geom* g, int mode, const float* src, const fl
                          float* tbuf, int64_t* idx_out, float* len_out) {
     int nd = g->ndim;
     int64_t nseg = 0;
+    if (mode == MODE_MARCH) return (nd == 2) ? trace_march2d_f32(g, src, dst, idx_out, len_out) : 0;
     if (mode == MODE_TORCH) {
         int64_t cnt = crossings_torch_f32(g, src, dst, tbuf);
         float vw[3];
@@ -645,6 +721,25 @@ int orc_compress_torch(const float* tvals, int64_t n_ray, int64_t K, const float
     return 0;
 }
 
+/* Decoded crossing of the CUDA consumers (3d.cu:126-129): the integer running
+ * sum k of the deltas, then start + k*scale contracted by nvcc into one fma. */
+static float decode_cuda(uint32_t k, float sc, float st) { return fmaf((float)k, sc, st); }
+
+/* The decoded table of the CUDA consumers: out[r][0] = start, out[r][i] =
+ * decode_cuda(sum of deltas 1..i). */
+int orc_decode_u16_cuda(const uint16_t* u, const float* start, const float* scale, int64_t n_ray, int64_t K,
+                        float* out) {
+    for (int64_t r = 0; r < n_ray; ++r) {
+        uint32_t ks = 0;
+        if (K > 0) out[r * K] = start[r];
+        for (int64_t i = 1; i < K; ++i) {
+            ks += (uint32_t)u[r * K + i];
+            out[r * K + i] = decode_cuda(ks, scale[r], start[r]);
+        }
+    }
+    return 0;
+}
+
 /* Compressed consumers (3d.cu:99-157, 215-269): t1 = start + k*scale (FMA),
  * pairs with seg < 1e-9 skipped; segment geometry in the torch op order. */
 static int64_t trace_u16_f32(const orc_geom* g, const float* src, const float* dst, const uint16_t* u,
@@ -656,7 +751,7 @@ static int64_t trace_u16_f32(const orc_geom* g, const float* src, const float* d
     int64_t nseg = 0;
     for (int64_t i = 0; i + 1 < K; ++i) {
         ks += (uint32_t)u[i + 1];
-        float t1 = fmaf((float)ks, sc, st);
+        float t1 = decode_cuda(ks, sc, st);
         float t0 = tprev;
         tprev = t1;
         float p0[3], p1[3], m[3], c[3], ss = 0.0f;

@@ -37,9 +37,29 @@ static int64_t dispatch(int mode, int zero_rule, const int* n, const float* minv
                   : run<ND, MODE_OTF, false>(g, src, dst, n_ray, cap, idx, len, counts);
 }
 
+// The reference's 2D on-the-fly march (MODE_MARCH): every in-grid step.
+static int64_t run_march(const Grid<2>& g, const float* src, const float* dst, int64_t n_ray, int64_t cap,
+                         int64_t* idx, float* len, int64_t* counts) {
+    int64_t worst = 0;
+    for (int64_t r = 0; r < n_ray; ++r) {
+        int64_t k = 0;
+        int64_t* io = idx + r * cap;
+        float* lo = len + r * cap;
+        march_ray_2d(g, src + r * 2, dst + r * 2, [&](uint32_t i, float l) {
+            if (k < cap) { io[k] = i; lo[k] = l; }
+            ++k;
+        });
+        counts[r] = k;
+        if (k > worst) worst = k;
+    }
+    return worst;
+}
+
 extern "C" int64_t host_trace(int ndim, int mode, int zero_rule, const int* n, const float* minv,
                               const float* b, const float* src, const float* dst, int64_t n_ray,
                               int64_t cap, int64_t* idx, float* len, int64_t* counts) {
+    if (ndim == 2 && mode == MODE_MARCH)
+        return run_march(load_grid<2>(n, minv, b, zero_rule), src, dst, n_ray, cap, idx, len, counts);
     if (ndim == 2) return dispatch<2>(mode, zero_rule, n, minv, b, src, dst, n_ray, cap, idx, len, counts);
     return dispatch<3>(mode, zero_rule, n, minv, b, src, dst, n_ray, cap, idx, len, counts);
 }

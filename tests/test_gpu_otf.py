@@ -83,7 +83,7 @@ def test_closed_forms(dev):
     img = np.zeros((32, 32), np.float32)
     img[16, :] = 1
     e2, z2 = _t(np.eye(2), dev), _t(np.zeros(2), dev)
-    for mode in (N.MODE_TORCH, N.MODE_OTF):
+    for mode in (N.MODE_TORCH, N.MODE_OTF, N.MODE_MARCH):
         out = _C.forward_project_2d_on_the_fly_cuda(_t(img, dev), _t([[16.0, -5.0]], dev), _t([[16.0, 37.0]], dev),
                                                     e2, z2, mode=mode).cpu().numpy()
         assert abs(out[0, 0] - 32.0) < 1e-4  # test_ct_projector_2d_line_integral.py:31-43,70
@@ -383,3 +383,43 @@ def test_auto_layout_new_rays_in_freed_block(dev, mname, mode):
     assert key_m is not None and key_m[0]() is tm
     again = _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
     assert torch.equal(again, got)
+
+
+def test_march_2d_matches_oracle(dev):
+    """The reference's 2D on-the-fly march (the _C default for 2D on-the-fly
+    calls, as in the reference's _C): every in-grid step bit-exact against the
+    oracle (index and length, zero and negative lengths included), forward and
+    back projections within 1e-5, and the hand-derived known answers."""
+    rng = np.random.default_rng(77)
+    for trial in range(8):
+        shape, M, minv, b, s = random_geometry(rng, 2, rotated=(trial % 4 == 3))
+        kind = ["far", "inside", "lattice", "distant"][trial % 4]
+        src, dst = random_rays(rng, shape, s, 3000, kind)
+        idx, ln, cnt = _C.trace_segments(shape, _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev),
+                                         mode=N.MODE_MARCH)
+        gi, gl, gc = idx.cpu().numpy(), ln.cpu().numpy(), cnt.cpu().numpy()
+        oi, ol, oc = O.trace(shape, minv, b, src, dst, mode="march")
+        assert np.array_equal(gc, oc)
+        for r in range(src.shape[0]):
+            k = int(oc[r])
+            assert np.array_equal(gi[r, :k], oi[r, :k]), (trial, r)
+            assert np.array_equal(gl[r, :k].view(np.int32), ol[r, :k].view(np.int32)), (trial, r)
+        img = rng.uniform(0, 1, shape).astype(np.float32)
+        got = _C.forward_project_2d_on_the_fly_cuda(_t(img, dev), _t(src, dev), _t(dst, dev), _t(minv, dev),
+                                                    _t(b, dev)).cpu().numpy()[0]
+        ref = O.forward(img, minv, b, src, dst, mode="march")
+        assert rel_err(got, ref) < 1e-5
+        y = rng.uniform(0, 1, (1, src.shape[0])).astype(np.float32)
+        back = _C.back_project_2d_on_the_fly_cuda(_t(y, dev), _t(src, dev), _t(dst, dev), _t(minv, dev),
+                                                  _t(b, dev), *shape).cpu().numpy()
+        refb = O.back(y, shape, minv, b, src, dst, mode="march")
+        assert np.all(np.abs(back - refb) <= 1e-5 * (np.abs(refb) + np.abs(refb).max()))
+    e2, z2 = _t(np.eye(2), dev), _t(np.zeros(2), dev)
+    one = np.zeros((8, 8), np.float32)
+    one[3, 2] = 1
+    got = _C.forward_project_2d_on_the_fly_cuda(_t(one, dev), _t([[3.7, 2.0]], dev), _t([[9.7, 2.0]], dev), e2, z2)
+    assert abs(float(got[0, 0]) + 0.2) < 1e-6  # the negative first step of a ray starting past a voxel centre
+    col5 = np.zeros((8, 8), np.float32)
+    col5[:, 5] = 1
+    got = _C.forward_project_2d_on_the_fly_cuda(_t(col5, dev), _t([[-2.0, 4.5]], dev), _t([[10.0, 4.5]], dev), e2, z2)
+    assert abs(float(got[0, 0]) - 8.0) < 1e-5  # column coordinate 4.5 walks column 5

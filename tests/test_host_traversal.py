@@ -34,7 +34,7 @@ def host():
         ln = np.zeros((N, cap), np.float32)
         cnt = np.zeros(N, np.int64)
         p = lambda a: a.ctypes.data_as(P)  # noqa: E731
-        L.host_trace(nd, {"torch": 0, "otf": 1}[mode], {"thresh": 0, "exact": 1}[zr], p(n), p(mi), p(bb), p(src),
+        L.host_trace(nd, {"torch": 0, "otf": 1, "march": 2}[mode], {"thresh": 0, "exact": 1}[zr], p(n), p(mi), p(bb), p(src),
                      p(dst), N, cap, p(idx), p(ln), p(cnt))
         return [(idx[r, :cnt[r]], ln[r, :cnt[r]]) for r in range(N)]
 
@@ -68,6 +68,34 @@ def test_plane_and_corner_rays(host, mode, nd):
     shape = (9, 9, 9) if nd == 3 else (13, 11)
     src, dst = lattice_rays(rng, nd, 20000)
     _check(host, shape, np.eye(nd), np.zeros(nd), src, dst, mode)
+
+
+def _check_march(host, shape, minv, b, src, dst):
+    """Every in-grid step of the 2D march (zero and negative lengths included), bit for bit."""
+    got = host(shape, minv, b, src, dst, "march", "thresh")
+    idx, ln, cnt = O.trace(shape, minv, b, src, dst, mode="march")
+    bad = [r for r in range(src.shape[0]) if cnt[r] != len(got[r][0]) or (got[r][0] != idx[r, :cnt[r]]).any()
+           or (got[r][1].view(np.int32) != ln[r, :cnt[r]].view(np.int32)).any()]
+    assert not bad, f"{len(bad)} rays differ (first {bad[:5]})"
+    return int(cnt.sum()), int((ln <= 0).sum() - (cnt.size * ln.shape[1] - cnt.sum()))
+
+
+def test_march_host_equals_oracle(host):
+    """The product's march_ray_2d (host compilation) against the oracle's
+    restatement of src/ct_projector_2d.cu:246-413: random maps and rays, and
+    half-integer lattice rays (entry coordinates at voxel edges and centres)."""
+    rng = np.random.default_rng(71)
+    steps = nonpos = 0
+    for trial in range(16):
+        shape, M, minv, b, s = random_geometry(rng, 2, rotated=(trial % 4 == 3))
+        kind = ["far", "inside", "lattice", "distant"][trial % 4]
+        src, dst = random_rays(rng, shape, s, 1500, kind)
+        a, z = _check_march(host, shape, minv, b, src, dst)
+        steps, nonpos = steps + a, nonpos + z
+    src, dst = lattice_rays(rng, 2, 20000)
+    a, z = _check_march(host, (13, 11), np.eye(2), np.zeros(2), src, dst)
+    steps, nonpos = steps + a, nonpos + z
+    assert steps > 100000 and nonpos > 0  # the comparison covers zero and negative steps
 
 
 @pytest.mark.parametrize("mode", ["torch", "otf"])
