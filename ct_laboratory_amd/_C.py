@@ -347,18 +347,21 @@ def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b, *, out=None):
     return out
 
 
-def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z, *, out=None):
-    """[B, n_x, n_y, n_z] back projection from a crossing table (src/ct_projector_3d.cu:435-462)."""
+def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z, *, out=None, accumulate: bool = False):
+    """[B, n_x, n_y, n_z] back projection from a crossing table (src/ct_projector_3d.cu:435-462).
+    ``accumulate`` adds into ``out`` instead of overwriting it."""
     name = "back_project_3d_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
+    acc = bool(accumulate) and out is not None
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_tvals", N.ptr(sino), batch, N.ptr(tvals), K, N.ptr(src), N.ptr(dst), n_ray,
-               N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out), N.stream_of(sino))
+               N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), int(acc),
+               N.ptr(out), N.stream_of(sino))
     return out
 
 

@@ -13,7 +13,7 @@ import torch
 from ._backend import check_backend
 from .ct_projector_3d_hip import back_project_3d_hip, forward_project_3d_hip
 
-__all__ = ["CTProjector3DFunction", "OnTheFlyOptions"]
+__all__ = ["CTProjector3DFunction", "CTBackProjector3DFunction", "OnTheFlyOptions"]
 
 
 class OnTheFlyOptions:
@@ -74,3 +74,21 @@ class CTProjector3DFunction(torch.autograd.Function):
         grad = back_project_3d_hip(grad_output, tv, src, dst, M, b, n_x, n_y, n_z, semantics=opts.semantics,
                                    det_shape=opts.det_shape, brick_plan=opts.plan)
         return grad, None, None, None, None, None, None, None, None, None
+
+
+class CTBackProjector3DFunction(torch.autograd.Function):
+    """apply(sinogram, projector): the back projection of a 3D projector module
+    as a differentiable operation, whose backward is the module's forward
+    projection.  The reference declares this class with an instance
+    constructor and static methods that read ``self.base_forward_projector``
+    (ct_laboratory/tomography/ct_projector_3d_autograd.py:176-186), so it cannot
+    run; here the projector is an argument."""
+
+    @staticmethod
+    def forward(ctx, sinogram, projector):
+        ctx.projector = projector
+        return projector.back_project(sinogram.contiguous())
+
+    @staticmethod
+    def backward(ctx, grad_volume):
+        return ctx.projector.forward_project(grad_volume.contiguous()), None
