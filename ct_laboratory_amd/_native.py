@@ -12,7 +12,10 @@ import threading
 
 import torch
 
-from .build import lib_path
+
+def lib_path() -> str:
+    """Path of the built library (ct_laboratory_amd/_lib/libctlab_hip.so, written by build.py)."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libctlab_hip.so")
 
 _lock = threading.Lock()
 _lib = None

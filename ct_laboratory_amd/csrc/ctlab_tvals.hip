@@ -104,7 +104,8 @@ __device__ __forceinline__ void stage_rows(RowStage<T, C>& st, const T* __restri
     __syncthreads();
 }
 
-// 64 bytes of each row per chunk (8.5 KB of LDS per block).  On a 1.6 GB 3D
+// 64 bytes of each row per chunk (17 KB of LDS per block: 256 rows x 17 floats,
+// or 256 rows x 34 uint16 entries).  On a 1.6 GB 3D
 // table, 32 and 64 bytes per row ran 1.19 and 1.41 ms, 128 bytes 1.67 ms and
 // 256 bytes 2.86 ms: smaller stages leave room for more blocks per CU.
 constexpr int kStageF32 = 16;
@@ -184,6 +185,8 @@ __global__ __launch_bounds__(kBlock) void k_back_tvals(const float* __restrict__
     const int64_t bi = blockIdx.y;
     const float val = (r < n_ray) ? sino[bi * n_ray + r] : 0.0f;
     const bool active = r < n_ray && val != 0.0f;
+    // a block without a nonzero value stages no rows (every lane gets the same answer)
+    if (!__syncthreads_or(active)) return;
     const Grid<ND> g = load_grid<ND>(dims.n, minv, b, ZERO_THRESH);
     float s[ND] = {}, d[ND] = {};
     if (active) load_ray_t<ND>(src, dst, r, s, d);
@@ -335,6 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_back_u16(const float* __restrict__ s
     const int64_t bi = blockIdx.y;
     const float val = (r < n_ray) ? sino[bi * n_ray + r] : 0.0f;
     const bool active = r < n_ray && val != 0.0f;
+    // a block without a nonzero value stages no rows (every lane gets the same answer)
+    if (!__syncthreads_or(active)) return;
     const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
     float s[3] = {}, d[3] = {};
     float st0 = 0.0f, sc = 0.0f;

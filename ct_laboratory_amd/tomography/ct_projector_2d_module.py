@@ -14,7 +14,9 @@ compute_intersections_2d drops them): the traversal yields exactly the table's
 segments, in the same order (the forward is bit-identical), without reading
 4 (n_row + n_col + 2) bytes of table per ray, and the back projection sums LDS
 tiles instead of adding each segment with a global atomic.  A table that is
-replaced or written in place is consumed as given.
+replaced or written in place, or whose rays or map (src, dst, M, b) are
+replaced or written in place, is consumed as given, as the reference would
+apply it to the current rays.
 """
 from __future__ import annotations
 
@@ -49,12 +51,24 @@ class CTProjector2DModule(torch.nn.Module):
         if precomputed_intersections:
             tvals = compute_intersections_2d_hip(self.n_row, self.n_col, self.M, self.b, self.src, self.dst)
         self.register_buffer("tvals", tvals)
-        self._own_table = None if tvals is None else (weakref.ref(tvals), tvals._version)
+        self._own_table = self._record(tvals)
+
+    def _record(self, t):
+        """The table and the geometry it was computed for: weak references to the
+        table, M, b, src and dst buffers with their version counters."""
+        if t is None:
+            return None
+        return tuple((weakref.ref(x), x._version) for x in (t, self.M, self.b, self.src, self.dst))
 
     def _own_untouched(self) -> bool:
+        """Whether the table is the one this module computed and neither it nor
+        the geometry it was computed for (M, b, src, dst: same tensors, unwritten)
+        has changed since; otherwise the table is consumed as given."""
         t = self.tvals
-        return t is not None and self._own_table is not None and self._own_table[0]() is t and \
-            t._version == self._own_table[1]
+        if t is None or self._own_table is None:
+            return False
+        return all(ref() is x and x._version == ver
+                   for (ref, ver), x in zip(self._own_table, (t, self.M, self.b, self.src, self.dst)))
 
     def __getstate__(self):
         # a weak reference does not pickle (torch.save, copy.deepcopy): keep whether the
@@ -67,7 +81,7 @@ class CTProjector2DModule(torch.nn.Module):
         own = state.pop("_own_table", False)
         self.__dict__.update(state)
         t = self._buffers.get("tvals")
-        self._own_table = (weakref.ref(t), t._version) if (own is True and t is not None) else None
+        self._own_table = self._record(t) if own is True else None
 
     def _rule(self):
         """(tvals, semantics, zero_rule) to project with: no table and the

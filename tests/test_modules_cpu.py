@@ -3,8 +3,6 @@ module projects on the fly only while its table is the one it computed and
 nothing has written to it, and that state survives deepcopy and pickling."""
 import copy
 import io
-import weakref
-
 import torch
 
 from ct_laboratory_amd import _native as N
@@ -16,7 +14,7 @@ def _module_with_own_table():
                             precomputed_intersections=False)
     t = torch.zeros(3, 18)  # stands in for the table the constructor computes on the GPU
     m.register_buffer("tvals", t)
-    m._own_table = (weakref.ref(t), t._version)
+    m._own_table = m._record(t)
     return m
 
 
@@ -29,6 +27,18 @@ def test_own_table_rule_and_edits():
     m2 = _module_with_own_table()
     m2.tvals = torch.zeros(3, 18)  # replaced: consumed as given
     assert m2._rule()[0] is m2.tvals
+    # the rays or the map written in place or replaced: the table is applied to them as given
+    m3 = _module_with_own_table()
+    with torch.no_grad():
+        m3.src[0, 0] = 2.0
+    assert m3._rule()[0] is m3.tvals
+    m4 = _module_with_own_table()
+    m4.M = torch.eye(2) * 2.0
+    assert m4._rule()[0] is m4.tvals
+    m5 = _module_with_own_table()
+    with torch.no_grad():
+        m5.b += 1.0
+    assert m5._rule()[0] is m5.tvals
 
 
 def test_own_table_survives_copies():
