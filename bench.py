@@ -196,6 +196,65 @@ def refcalls_bench(a, dev) -> int:
     return 0
 
 
+def generic_bench(a, dev) -> int:
+    """The 256^3 cone-beam workload's rays in a seeded random order, so that no
+    detector layout describes them: the brick lists' build time, the back
+    projection of a dense sinogram over brick lists and with one ray per lane
+    (global atomics), the forward in ray order, and the adjoint of the list
+    pair.  Prints one JSON line."""
+    import torch
+
+    from ct_laboratory_amd import _C
+    from ct_laboratory_amd import _native as N
+
+    _, _, _, _, _, ts, td, tv, tm, tb = build_inputs(dev)
+    n = CFG["n"]
+    n_ray = int(ts.shape[0])
+    perm = torch.randperm(n_ray, generator=torch.Generator().manual_seed(5)).to(dev)
+    ts, td = ts[perm].contiguous(), td[perm].contiguous()
+    del perm
+    y = torch.from_numpy(np.random.default_rng(2).standard_normal(n_ray, dtype=np.float32)).to(dev)[None]
+    mode = N.MODE_TORCH
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps, out
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bins = _C.brick_bins(ts, td, tm, tb, (n, n, n), mode=mode)
+    torch.cuda.synchronize()
+    t_bins = time.perf_counter() - t0
+    ws = torch.empty(1, device=dev)
+    back_ms, bk = timed(lambda: _C.back_project_3d_brick_bins(y, ts, td, bins, tm, tb, n, n, n, mode=mode,
+                                                                 workspace=ws), a.steps)
+    fwd_ms, sino = timed(lambda: _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb, mode=mode, det_shape=None),
+                         max(1, a.steps // 2))
+    _C.AUTO_LAYOUT = False
+    try:
+        atomic_ms, _ = timed(lambda: _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, n, n, n, mode=mode), 1)
+    finally:
+        _C.AUTO_LAYOUT = True
+    lhs = float(torch.dot(sino[0].double(), y[0].double()))
+    rhs = float(torch.dot(tv.reshape(-1).double(), bk.reshape(-1).double()))
+    res = {
+        "metric": "256^3 cone-beam rays in random order (no detector layout): back projection over brick lists",
+        "workload": "720 views x 384x384 rays at 1.5 mm, SID 1000, SDD 1500, shuffled (seed 5); dense y ~ N(0,1)",
+        "n_ray": n_ray, "pairs_listed": int(bins.ids.numel()), "pairs_per_ray": bins.ids.numel() / n_ray,
+        "bins_build_s": t_bins, "back_bins_ms": back_ms, "back_atomic_ms": atomic_ms, "forward_ray_order_ms": fwd_ms,
+        "adjoint_rel_err": abs(lhs - rhs) / abs(lhs),
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def batched_bench(a, dev) -> int:
     """64 independent 2D scans (sizes 128^2-512^2, 90-720 views, half fan beam, half
     StaticCT ring; scan i seeded by default_rng(100 + i)): forward + back of all
@@ -273,15 +332,19 @@ def serve_bench(a, dev) -> int:
     """Served MAP stream: 70% fan-beam (256^2-512^2, 360-720 views, 2n bins), 30%
     cone-beam (128^3-256^3, 180-720 views, 256^2-384^2 panel), 20 iterations each
     with the quadratic prior (weight 1e-2), Poisson arrivals; admission by a 200 GB
-    HBM budget.  Prints one JSON line."""
+    HBM budget.  The server runs first, then the plain loop; each phase is
+    bracketed by a short spin kernel so that a kernel trace of this run splits
+    into the two phases.  Prints one JSON line."""
     import torch
 
     from ct_laboratory_amd.serving import ReconServer, build_problem, latency_stats, plain_loop, poisson_workload
 
     reqs = poisson_workload(a.requests, a.qps)
     print(f"[serve] building {len(reqs)} problems", file=sys.stderr, flush=True)
-    for r in reqs:
+    for i, r in enumerate(reqs):
         build_problem(r, dev)
+        if i % 20 == 19:
+            print(f"[serve] {i + 1} problems built", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     # warm the library and torch kernels once on a small request
     from ct_laboratory_amd.serving import MAPRequest
@@ -290,10 +353,19 @@ def serve_bench(a, dev) -> int:
         build_problem(r, dev)
     ReconServer(dev, slots=2).run(warm)
     plain_loop(warm, dev)
+    def mark():
+        # a spin kernel of ~1000 cycles marks a phase boundary in a kernel trace
+        torch.cuda.synchronize()
+        torch.cuda._sleep(1000)
+        torch.cuda.synchronize()
+
     print("[serve] server", file=sys.stderr, flush=True)
+    mark()
     served, span_s = ReconServer(dev, hbm_budget=200e9, slots=a.slots).run(reqs)
+    mark()
     print("[serve] plain loop", file=sys.stderr, flush=True)
     plain, span_p = plain_loop(reqs, dev)
+    mark()
 
     def rmse(res):
         e = [float(((res[r.rid][0] - r.x_true) ** 2).mean().sqrt()) for r in reqs]
@@ -397,6 +469,45 @@ def map_bench(a, dev) -> int:
     return 0
 
 
+def trace_summary(path: str) -> int:
+    """Device-busy breakdown of a rocprofv3 kernel trace (its *_results.db) of
+    a ``--serve`` run: the run's phases are delimited by the spin kernels it
+    launches between them; per phase, the wall time from marker to marker and
+    the time with at least one kernel running (kernel intervals merged), with
+    the busy time split by kernel name.  Prints one JSON line."""
+    import sqlite3
+    from collections import defaultdict
+
+    con = sqlite3.connect(path)
+    rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    marks = [r for r in rows if "spin" in r[0].lower() or "sleep" in r[0].lower()]
+    if len(marks) < 3:
+        print(json.dumps({"error": f"expected 3 phase markers, found {len(marks)}"}))
+        return 1
+    phases = {"server": (marks[0][2], marks[1][1]), "plain": (marks[1][2], marks[2][1])}
+    out = {}
+    for name, (t0, t1) in phases.items():
+        ks = [r for r in rows if r[1] >= t0 and r[2] <= t1]
+        busy, cur_s, cur_e = 0, None, None
+        by_kernel = defaultdict(int)
+        for kname, s0, s1 in ks:
+            by_kernel[kname.split("(")[0].replace("void ", "")[:60]] += s1 - s0
+            if cur_e is None or s0 > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = s0, s1
+            else:
+                cur_e = max(cur_e, s1)
+        if cur_e is not None:
+            busy += cur_e - cur_s
+        top = sorted(by_kernel.items(), key=lambda kv: -kv[1])[:8]
+        out[name] = {"wall_s": (t1 - t0) * 1e-9, "busy_s": busy * 1e-9, "busy_frac": busy / max(t1 - t0, 1),
+                     "kernels": len(ks), "kernel_time_s": sum(by_kernel.values()) * 1e-9,
+                     "top_kernels_s": {k: v * 1e-9 for k, v in top}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, a) -> float | None:
     """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the same counters
@@ -435,15 +546,21 @@ def main(argv=None) -> int:
                     help="the 2D fan-beam configuration (256^2, 360 views x 512 bins) with its CPU baseline")
     ap.add_argument("--refcalls", action="store_true",
                     help="the 256^3 cone-beam workload through the _C calls as the reference's Python makes them")
+    ap.add_argument("--generic", action="store_true",
+                    help="the 256^3 workload's rays in random order: back projection over brick lists vs atomics")
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven 2D scans: one persistent launch (autotuned) vs 64 sequential launches")
     ap.add_argument("--map", action="store_true",
                     help="MAP iterations on the 512x512x256 / 600-view cone-beam problem (fused graph vs autograd loop); one JSON line")
     ap.add_argument("--map-iters", type=int, default=6)
-    ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--qps", type=float, default=8.0)
+    ap.add_argument("--trace-summary", default=None, metavar="DB",
+                    help="device-busy breakdown of a rocprofv3 kernel-trace database of a --serve run; one JSON line")
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--slots", type=int, default=8)
     a = ap.parse_args(argv)
+    if a.trace_summary:
+        return trace_summary(a.trace_summary)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projector shards over rays "
               "across GPUs by design but no multi-GPU run is part of this build.", file=sys.stderr)
@@ -461,6 +578,8 @@ def main(argv=None) -> int:
         return map_bench(a, dev)
     if a.batched:
         return batched_bench(a, dev)
+    if a.generic:
+        return generic_bench(a, dev)
     if a.fan2d:
         return fan2d_bench(a, dev)
     if a.refcalls:

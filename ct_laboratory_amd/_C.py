@@ -138,8 +138,13 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
     """[B, n_x, n_y, n_z] back projection (src/ct_projector_3d.cu:1051-1079).
 
     Without ``det_shape``, rays recognised as a stack of flat-panel views are
-    back-projected on LDS bricks (same segments); ``det_shape=(rows, cols)``
-    declares the layout and keeps the one-ray-per-lane kernel, scheduled by it."""
+    back-projected on LDS bricks over their detector rectangles, and other ray
+    sets on LDS bricks over per-brick ray lists built once per ray set (same
+    segments either way); ``det_shape=(rows, cols)`` declares the layout and
+    keeps the one-ray-per-lane kernel, scheduled by it.  Under graph capture the
+    caches are not consulted (a lookup may synchronise) and the one-ray-per-lane
+    kernel runs; a caller that captures passes its plan or lists to
+    back_project_3d_brick / back_project_3d_brick_bins."""
     name = "back_project_3d_on_the_fly_cuda"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
@@ -150,6 +155,9 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
         plan = _plan_of(src, dst, layout, M_inv, b, (int(n_x), int(n_y), int(n_z))) if layout else None
         if plan is not None:
             return back_project_3d_brick(sino, src, dst, layout, plan, M_inv, b, n_x, n_y, n_z, mode=mode, out=out)
+        bins = _bins_of(src, dst, M_inv, b, (int(n_x), int(n_y), int(n_z)), mode)
+        if bins is not None:
+            return back_project_3d_brick_bins(sino, src, dst, bins, M_inv, b, n_x, n_y, n_z, mode=mode, out=out)
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
@@ -582,6 +590,76 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
 
 
 # ---------------------------------------------------------------------------
+# Brick lists: rays in any order back-projected brick by brick (no reference counterpart)
+# ---------------------------------------------------------------------------
+
+
+class BrickBins:
+    """Per-brick ray lists of a ray set: offsets uint32 [n_brick + 1] (as int32
+    storage), ids uint32 [total] (as int32), emax float32 [1] (the largest
+    midpoint error bound), and the grid and rule they were built for."""
+
+    def __init__(self, offsets, ids, emax, shape, mode, n_ray):
+        self.offsets, self.ids, self.emax = offsets, ids, emax
+        self.shape, self.mode, self.n_ray = tuple(int(x) for x in shape), int(mode), int(n_ray)
+
+
+def brick_bins(src, dst, M_inv, b, shape, *, mode: int = N.MODE_TORCH):
+    """Builds the brick lists of a ray set (two passes and one synchronisation
+    to size the id list)."""
+    name = "brick_bins"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 3)
+    _affine(name, M_inv, b, 3)
+    _require(n_ray < 2 ** 32, f"{name}: at most 2^32 - 1 rays")
+    nx, ny, nz = (int(x) for x in shape)
+    L = N.load()
+    nbk = int(L.ctlab_brick_count_3d(nx, ny, nz))
+    _require(nbk > 0, f"{name}: bad grid")
+    dev = src.device
+    counts = torch.empty((nbk,), device=dev, dtype=torch.int32)
+    emax = torch.empty((1,), device=dev, dtype=torch.float32)
+    with torch.cuda.device(dev):
+        N.call("ctlab_brick_bins_count_3d", N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), nx, ny, nz, int(mode),
+               N.ptr(counts), N.ptr(emax), N.stream_of(src))
+        offsets = torch.zeros((nbk + 1,), device=dev, dtype=torch.int64)
+        torch.cumsum(counts.to(torch.int64) & 0xFFFFFFFF, 0, out=offsets[1:])
+        total = int(offsets[-1])
+        _require(total < 2 ** 32, f"{name}: {total} (brick, ray) pairs exceed 2^32 - 1")
+        off32 = offsets.to(torch.int32)  # uint32 values in int32 storage
+        cursor = off32[:-1].clone()
+        ids = torch.full((max(total, 1),), -1, device=dev, dtype=torch.int32)  # 0xffffffff: no ray
+        N.call("ctlab_brick_bins_fill_3d", N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), nx, ny, nz, int(mode),
+               N.ptr(cursor), N.ptr(ids), total, N.stream_of(src))
+    return BrickBins(off32, ids, emax, shape, mode, n_ray)
+
+
+def back_project_3d_brick_bins(sino, src, dst, bins, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH, out=None,
+                               accumulate: bool = False, workspace=None):
+    """[B, n_x, n_y, n_z] back projection of rays in any order by LDS-resident
+    bricks over the lists ``bins`` (from brick_bins for these rays, grid and
+    rule): the segments of back_project_3d_on_the_fly_cuda, no global atomics."""
+    name = "back_project_3d_brick_bins"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    _require(isinstance(bins, BrickBins) and bins.shape == (int(n_x), int(n_y), int(n_z)) and bins.n_ray == n_ray and
+             bins.mode == int(mode), f"{name}: the brick lists were built for another grid, rule or ray count")
+    if out is None:
+        accumulate = False
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
+    if workspace is None:
+        workspace = torch.empty(batch, device=sino.device, dtype=torch.float32)
+    _require(workspace.dtype == torch.float32 and workspace.is_cuda and workspace.numel() >= batch,
+             f"{name}: workspace must hold batch floats")
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_brick_bins", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(bins.offsets), N.ptr(bins.ids), N.ptr(bins.emax), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y),
+               int(n_z), int(mode), int(bool(accumulate)), N.ptr(out), N.ptr(workspace), N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Detector layout of 3D rays passed without one
 # ---------------------------------------------------------------------------
 
@@ -711,6 +789,32 @@ def _layout_of(src, dst):
     with _CACHE_LOCK:
         _remember(_LAYOUTS, key, (weakref.ref(src), weakref.ref(dst), layout))
     return layout
+
+
+_BINS: OrderedDict = OrderedDict()  # (ray key, grid, rule, M_inv and b values) -> (src ref, dst ref, BrickBins)
+_BINS_CACHE_SIZE = 2
+_MIN_BIN_RAYS = 4096  # below this the one-ray-per-lane kernel is as fast
+
+
+def _bins_of(src, dst, M_inv, b, shape, mode):
+    """Brick lists of a ray set without a flat-panel layout, cached like the
+    brick plans; None while a graph is being captured (building them
+    synchronises) or for very small ray sets."""
+    if not AUTO_LAYOUT or _capturing() or int(src.shape[0]) < _MIN_BIN_RAYS:
+        return None
+    key = (_ray_key(src, dst), shape, int(mode), _map_values(M_inv), _map_values(b))
+    with _CACHE_LOCK:
+        e = _BINS.get(key)
+        if e is not None and _alive(e[0], src) and _alive(e[1], dst):
+            _BINS.move_to_end(key)
+            return e[2]
+    bins = brick_bins(src, dst, M_inv, b, shape, mode=mode)
+    with _CACHE_LOCK:
+        _BINS[key] = (weakref.ref(src), weakref.ref(dst), bins)
+        _BINS.move_to_end(key)
+        while len(_BINS) > _BINS_CACHE_SIZE:
+            _BINS.popitem(last=False)
+    return bins
 
 
 def _plan_of(src, dst, layout, M_inv, b, shape):

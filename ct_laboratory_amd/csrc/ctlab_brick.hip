@@ -878,3 +878,521 @@ extern "C" int ctlab_back_project_3d_brick_noalloc(const float* sino, int64_t ba
     return back_brick(fn, sino, batch, src, dst, n_view, det_rows, det_cols, plan, M_inv, b, n_x, n_y, n_z, mode,
                       accumulate, out, view_max, stream);
 }
+
+// ===========================================================================
+// Rays in any order: brick lists
+// ===========================================================================
+//
+// Without a detector layout a brick's rays cannot be read off a shadow
+// rectangle.  A binning pass lists instead, for every brick, the rays that can
+// have a segment in it: those passing through the brick's cells widened by
+// E + 0.05 voxel, with E the ray's own midpoint error bound, which is the test
+// the filing pass above applies with a bound over views.  A list variant of the
+// brick kernel then traces the rays of each list exactly as the shadow variant
+// traces the rays of its rectangles.  The lists depend only on the geometry
+// (rays, grid, map), so they are built once per ray set.
+
+namespace ctlab {
+
+constexpr int kBinThreads = 256;
+constexpr int64_t kBinRaysPerBlock = 8192;   // rays per block of the fill pass
+constexpr int kBinMaxLdsBricks = 16384;      // bricks counted in LDS (64 KB per block)
+
+__device__ __forceinline__ int floor_div_clamped(float c, int B, int nb) {
+    const float q = floorf(c / (float)B);
+    return (int)fminf(fmaxf(q, 0.0f), (float)(nb - 1));
+}
+
+// Calls emit(brick) for every brick whose cells, widened by m = E + 0.05, the
+// ray's segment t in [0, 1] passes through (slab_estimate > 0), E being the
+// ray's own midpoint error bound as trace_ray_window computes it.  Returns E,
+// or -1 for a ray the traversal gives no segment (non-finite input, E above 8,
+// or a zero-length ray under the otf rule).  Candidates are enumerated layer by
+// layer along the axis the ray crosses most bricks of; within a layer (widened
+// by m + 1 voxel) the other two axes span one or two bricks each.
+template <class Emit>
+__device__ float ray_bricks(const Grid<3>& g, const Dims3& dims, const BrickGrid& bg, int mode,
+                            const float* __restrict__ src, const float* __restrict__ dst, Emit&& emit) {
+    if (!g.finite) return -1.0f;
+    bool fin = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fin = fin && (fabsf(src[a]) <= 3.0e38f) && (fabsf(dst[a]) <= 3.0e38f);
+    if (!fin) return -1.0f;
+    if (mode == MODE_OTF) {
+        const float v0 = dst[0] - src[0], v1 = dst[1] - src[1], v2 = dst[2] - src[2];
+        const float rl = __builtin_sqrtf(__builtin_fmaf(v2, v2, __builtin_fmaf(v1, v1, v0 * v0)));
+        if (rl < 1.0e-12f) return -1.0f;
+    }
+    float sv[3], dv[3];
+    to_voxel<3, MODE_TORCH>(g, src, sv);
+    to_voxel<3, MODE_TORCH>(g, dst, dv);
+    float E = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float mag = fabsf(sv[a]) + fabsf(dv[a]) + (float)g.n[a] + 1.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) mag += fabsf(g.minv[a * 3 + j]) * (fabsf(src[j]) + fabsf(dst[j]) + fabsf(g.b[j]));
+        const float Ea = mag * 7.62939453125e-06f;  // 2^-17
+        if (!(Ea <= 8.0f)) return -1.0f;
+        E = fmaxf(E, Ea);
+    }
+    const float m = E + 0.05f, w = m + 1.0f;
+    const int B[3] = {kBrickX, kBrickY, kBrickZ};
+    const int nb[3] = {bg.nbx, bg.nby, bg.nbz};
+    float dir[3];
+    float t0 = 0.0f, t1 = 1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        dir[a] = dv[a] - sv[a];
+        const float lo = -0.5f - w, hi = (float)dims.n[a] - 0.5f + w;
+        if (dir[a] == 0.0f) {
+            if (!(sv[a] >= lo && sv[a] <= hi)) return E;
+            continue;
+        }
+        const float ta = (lo - sv[a]) / dir[a], tb = (hi - sv[a]) / dir[a];
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    }
+    if (!(t0 <= t1)) return E;
+    int D = 0;
+    float best = -1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float r = fabsf(dir[a]) / (float)B[a];
+        if (r > best) {
+            best = r;
+            D = a;
+        }
+    }
+    const int A1 = (D == 0) ? 1 : 0, A2 = (D == 2) ? 1 : 2;
+    const float cA = sv[D] + t0 * dir[D], cB = sv[D] + t1 * dir[D];
+    const int k0 = floor_div_clamped(fminf(cA, cB) + 0.5f - w, B[D], nb[D]);
+    const int k1 = floor_div_clamped(fmaxf(cA, cB) + 0.5f + w, B[D], nb[D]);
+    for (int k = k0; k <= k1; ++k) {
+        float ta = t0, tb = t1;
+        if (dir[D] != 0.0f) {
+            const float lo = (float)(k * B[D]) - 0.5f - w, hi = (float)((k + 1) * B[D]) - 0.5f + w;
+            const float x0 = (lo - sv[D]) / dir[D], x1 = (hi - sv[D]) / dir[D];
+            ta = fmaxf(t0, fminf(x0, x1));
+            tb = fminf(t1, fmaxf(x0, x1));
+            if (!(ta <= tb)) continue;
+        }
+        int j0[2], j1[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = q ? A2 : A1;
+            const float ca = sv[a] + ta * dir[a], cb = sv[a] + tb * dir[a];
+            j0[q] = floor_div_clamped(fminf(ca, cb) + 0.5f - w, B[a], nb[a]);
+            j1[q] = floor_div_clamped(fmaxf(ca, cb) + 0.5f + w, B[a], nb[a]);
+        }
+        for (int i1 = j0[0]; i1 <= j1[0]; ++i1) {
+            for (int i2 = j0[1]; i2 <= j1[1]; ++i2) {
+                int bi[3];
+                bi[D] = k;
+                bi[A1] = i1;
+                bi[A2] = i2;
+                float blo[3], bhi[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int lo = bi[a] * B[a];
+                    const int ext = min(B[a], dims.n[a] - lo);
+                    blo[a] = (float)lo - 0.5f - m;
+                    bhi[a] = (float)(lo + ext) - 0.5f + m;
+                }
+                if (slab_estimate(sv, dv, blo, bhi, 0.0f))
+                    emit((uint32_t)((bi[0] * nb[1] + bi[1]) * nb[2] + bi[2]));
+            }
+        }
+    }
+    return E;
+}
+
+template <int ND>
+__device__ __forceinline__ void load_ray3(const float* __restrict__ src, const float* __restrict__ dst, int64_t r,
+                                          float* s, float* d) {
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        s[a] = src[3 * r + a];
+        d[a] = dst[3 * r + a];
+    }
+}
+
+// Pass 1: the number of rays per brick (LDS histogram per block when the grid
+// has few enough bricks) and the largest E over the rays with segments.
+template <bool LDS>
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(const float* __restrict__ src,
+                                                            const float* __restrict__ dst, int64_t n_ray,
+                                                            Dims3 dims, BrickGrid bg,
+                                                            const float* __restrict__ minv,
+                                                            const float* __restrict__ b, int mode,
+                                                            uint32_t* __restrict__ counts,
+                                                            unsigned int* __restrict__ emax_bits) {
+    extern __shared__ uint32_t hist[];
+    const int nbk = bg.nbx * bg.nby * bg.nbz;
+    if (LDS) {
+        for (int i = threadIdx.x; i < nbk; i += kBinThreads) hist[i] = 0u;
+        __syncthreads();
+    }
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    float emax = 0.0f;
+    for (int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x; r < n_ray;
+         r += (int64_t)gridDim.x * kBinThreads) {
+        float s[3], d[3];
+        load_ray3<3>(src, dst, r, s, d);
+        const float e = ray_bricks(g, dims, bg, mode, s, d, [&](uint32_t bk) {
+            if (LDS)
+                atomicAdd(&hist[bk], 1u);
+            else
+                atomicAdd(&counts[bk], 1u);
+        });
+        emax = fmaxf(emax, e);
+    }
+    for (int o = 32; o > 0; o >>= 1) emax = fmaxf(emax, __shfl_xor(emax, o));
+    if (threadIdx.x % 64 == 0) atomicMax(emax_bits, __float_as_uint(emax));  // emax >= 0: bits order as values
+    if (LDS) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nbk; i += kBinThreads)
+            if (hist[i]) atomicAdd(&counts[i], hist[i]);
+    }
+}
+
+// Pass 2: each block lists the rays of its range.  It counts its rays per
+// brick in LDS, reserves one contiguous run per brick with a single global
+// atomic, and places the ray ids with LDS cursors.  The order of ids within a
+// brick's list depends on timing; the sums the brick kernel forms from them do
+// not (fixed point).
+template <bool LDS>
+__global__ __launch_bounds__(kBinThreads) void k_bin_fill(const float* __restrict__ src,
+                                                           const float* __restrict__ dst, int64_t n_ray,
+                                                           Dims3 dims, BrickGrid bg,
+                                                           const float* __restrict__ minv,
+                                                           const float* __restrict__ b, int mode,
+                                                           uint32_t* __restrict__ cursor,
+                                                           uint32_t* __restrict__ ids, int64_t n_ids) {
+    extern __shared__ uint32_t loc[];
+    const int nbk = bg.nbx * bg.nby * bg.nbz;
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const int64_t r0 = (int64_t)blockIdx.x * kBinRaysPerBlock;
+    const int64_t r1 = min(n_ray, r0 + kBinRaysPerBlock);
+    if (LDS) {
+        for (int i = threadIdx.x; i < nbk; i += kBinThreads) loc[i] = 0u;
+        __syncthreads();
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += kBinThreads) {
+            float s[3], d[3];
+            load_ray3<3>(src, dst, r, s, d);
+            ray_bricks(g, dims, bg, mode, s, d, [&](uint32_t bk) { atomicAdd(&loc[bk], 1u); });
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nbk; i += kBinThreads) {
+            const uint32_t c = loc[i];
+            if (c) loc[i] = atomicAdd(&cursor[i], c);
+        }
+        __syncthreads();
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += kBinThreads) {
+            float s[3], d[3];
+            load_ray3<3>(src, dst, r, s, d);
+            ray_bricks(g, dims, bg, mode, s, d, [&](uint32_t bk) {
+                const uint32_t slot = atomicAdd(&loc[bk], 1u);
+                if ((int64_t)slot < n_ids) ids[slot] = (uint32_t)r;
+            });
+        }
+    } else {
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += kBinThreads) {
+            float s[3], d[3];
+            load_ray3<3>(src, dst, r, s, d);
+            ray_bricks(g, dims, bg, mode, s, d, [&](uint32_t bk) {
+                const uint32_t slot = atomicAdd(&cursor[bk], 1u);
+                if ((int64_t)slot < n_ids) ids[slot] = (uint32_t)r;
+            });
+        }
+    }
+}
+
+// Largest |value| of each batch element's sinogram (INF when one is INF or
+// NaN), as the bits of a non-negative float: max_bits must hold zeros.
+__global__ __launch_bounds__(256) void k_absmax_all(const float* __restrict__ sino, int64_t n_ray,
+                                                    unsigned int* __restrict__ max_bits) {
+    const float* __restrict__ s = sino + (int64_t)blockIdx.y * n_ray;
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n_ray; i += (int64_t)gridDim.x * 256) {
+        const float a = fabsf(s[i]);
+        m = fmaxf(m, (a <= 3.0e38f) ? a : __builtin_huge_valf());
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x % 64 == 0) atomicMax(max_bits + blockIdx.y, __float_as_uint(m));
+}
+
+// The rays of a brick list, in chunks as brick_items takes the rays of its
+// shadow rectangles: zero-valued rays and rays that miss the widened brick are
+// dropped, the rest filed by estimated crossing count and traced in that order.
+template <int MODE, bool DIAG, class Add>
+__device__ __forceinline__ void list_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
+                                           const float* __restrict__ src, const float* __restrict__ dst,
+                                           const uint32_t* __restrict__ ids, int64_t n_ray, uint32_t item_lo,
+                                           uint32_t item_hi, float e_brick, float s1, float s2, uint16_t* queue,
+                                           uint32_t* bcnt, Add&& add) {
+    const int tid = threadIdx.x;
+    const bool e_ok = e_brick <= 8.0f;
+    const float e_hint = e_ok ? e_brick : -1.0f;
+    const float m = (e_ok ? e_brick : 8.0f) + 0.05f;
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = (float)win.lo[a] - 0.5f - m;
+        bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
+    }
+    for (uint32_t c0 = item_lo; c0 < item_hi; c0 += (uint32_t)kChunk) {
+        const uint32_t c1 = min(c0 + (uint32_t)kChunk, item_hi);
+        if (tid <= kBuckets) bcnt[tid] = 0u;
+        __syncthreads();
+        uint32_t key[kPerThread];
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            key[k] = ~0u;
+            const uint32_t item = c0 + (uint32_t)tid + (uint32_t)(k * kBrickThreads);
+            if (item < c1) {
+                const uint32_t ray = ids[item];  // unfilled slots hold 0xffffffff
+                if ((int64_t)ray < n_ray && sn[ray] != 0.0f) {
+                    const uint32_t est = segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
+                    if (est) {
+                        const uint32_t l2 = 31u - (uint32_t)__clz(est);
+                        const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
+                        key[k] = (bk << 12) | atomicAdd(&bcnt[bk], 1u);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kBuckets; ++k) {
+                const uint32_t c = bcnt[k];
+                bcnt[k] = run;
+                run += c;
+            }
+            bcnt[kBuckets] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k)
+            if (key[k] != ~0u) queue[bcnt[key[k] >> 12] + (key[k] & 0xfffu)] = (uint16_t)(tid + k * kBrickThreads);
+        __syncthreads();
+        const uint32_t hits = bcnt[kBuckets];
+        for (uint32_t q = tid; q < hits; q += kBrickThreads) {
+            const uint32_t ray = ids[c0 + queue[q]];
+            const float val = sn[ray] * s1 * s2;
+            float s[3], d[3];
+            for (int a = 0; a < 3; ++a) {
+                s[a] = src[3ull * ray + a];
+                d[a] = dst[3ull * ray + a];
+            }
+            trace_ray_window<3, MODE, DIAG, false>(
+                g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
+        }
+        __syncthreads();
+    }
+}
+
+// One work unit of the list kernel: part `part` of brick `bid` (the part's
+// share of the brick's list) for batch element bi.  The value bound is the
+// batch element's largest |value| and the count the list's length; otherwise
+// as brick_unit.
+template <int MODE>
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick_list(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    BrickGrid bg, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ids,
+    const float* __restrict__ emax_in, const float* __restrict__ src, const float* __restrict__ dst,
+    const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ vmax_in, int accumulate,
+    int split) {
+    __shared__ unsigned long long acc[kBrickLds];
+    __shared__ __align__(16) uint16_t queue[kChunk];
+    __shared__ uint32_t bcnt[kBuckets + 1];
+    const int tid = threadIdx.x;
+    const int bid = (int)(blockIdx.x / (uint32_t)split), part = (int)(blockIdx.x % (uint32_t)split);
+    const int64_t bi = blockIdx.y;
+    const Window<3> win = brick_window(dims, bg, bid);
+    const uint32_t base = offsets[bid], total = offsets[bid + 1] - base;
+    const uint32_t item_lo = base + (uint32_t)((uint64_t)total * part / split);
+    const uint32_t item_hi = base + (uint32_t)((uint64_t)total * (part + 1) / split);
+    float* __restrict__ o = out + bi * vol_elems;
+    const int ey = win.ext[1], ez = win.ext[2];
+    const int nvox = win.ext[0] * ey * ez;
+    const float vmax = (total > 0) ? vmax_in[bi] : 0.0f;
+    const float emax = emax_in[0];
+    const float e_brick = (emax >= 0.0f && emax <= 3.0e38f) ? emax * 1.0009765625f : __builtin_huge_valf();
+    if (vmax == 0.0f) {
+        if (split == 1 && !accumulate) {  // split > 1: the host zeroed out
+            for (int i = tid; i < nvox; i += kBrickThreads) {
+                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
+            }
+        }
+        return;
+    }
+    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const float* __restrict__ sn = sino + bi * n_ray;
+    // power-of-two scale as in brick_unit, with the list's length as the count
+    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
+    const bool fixed = g.finite && cmax <= 1.0e38;
+    const double bound = cmax * (double)(total > 0 ? total : 1);
+    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;
+    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;
+    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
+    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);
+    __syncthreads();
+    if (fixed) {
+        auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
+        if (g.diag)
+            list_items<MODE, true>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
+        else
+            list_items<MODE, false>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
+    } else {
+        if (split == 1 && !accumulate) {
+            for (int i = tid; i < nvox; i += kBrickThreads) {
+                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
+            }
+        }
+        __syncthreads();
+        auto add = [&](uint32_t li, float c) {
+            const uint32_t r = li % kStrideX;
+            const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
+            const int64_t gi =
+                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+            unsafeAtomicAdd(o + gi, c);
+        };
+        if (g.diag)
+            list_items<MODE, true>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
+        else
+            list_items<MODE, false>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
+        return;
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -F);
+    for (int i = tid; i < nvox; i += kBrickThreads) {
+        const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+        const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
+        if (split > 1)
+            add_f32_ieee(o + gi, v);
+        else
+            o[gi] = accumulate ? o[gi] + v : v;
+    }
+}
+
+static int check_grid3(const char* fn, const int64_t* n) {
+    int64_t V = 1;
+    for (int a = 0; a < 3; ++a) {
+        CTLAB_REQUIRE(n[a] >= 1, "%s: bad grid dimension", fn);
+        V *= n[a];
+    }
+    CTLAB_REQUIRE(V <= kMaxVoxels, "%s: grid has more than 2^31-1 voxels", fn);
+    return check_index_limits(fn, n, 3);
+}
+
+}  // namespace ctlab
+
+extern "C" int64_t ctlab_brick_count_3d(int64_t n_x, int64_t n_y, int64_t n_z) {
+    using namespace ctlab;
+    const int64_t n[3] = {n_x, n_y, n_z};
+    if (n_x < 1 || n_y < 1 || n_z < 1) return -1;
+    const BrickGrid bg = brick_grid(n);
+    return (int64_t)bg.nbx * bg.nby * bg.nbz;
+}
+
+extern "C" int ctlab_brick_bins_count_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                         const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
+                                         uint32_t* counts, float* emax, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "brick_bins_count_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
+    CTLAB_REQUIRE(counts && emax && (n_ray == 0 || (src && dst && M_inv && b)), "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    st = fill_f32(reinterpret_cast<float*>(counts), nbk, 0.0f, as_stream(stream));  // zero bits
+    if (st) return st;
+    st = fill_f32(emax, 1, 0.0f, as_stream(stream));
+    if (st || n_ray == 0) return st;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    const unsigned blocks = (unsigned)std::min<int64_t>((n_ray + kBinThreads - 1) / kBinThreads, 4096);
+    if (nbk <= kBinMaxLdsBricks)
+        hipLaunchKernelGGL(k_bin_count<true>, dim3(blocks), dim3(kBinThreads), (size_t)nbk * 4, as_stream(stream), src,
+                           dst, n_ray, d, bg, M_inv, b, mode, counts, reinterpret_cast<unsigned int*>(emax));
+    else
+        hipLaunchKernelGGL(k_bin_count<false>, dim3(blocks), dim3(kBinThreads), 0, as_stream(stream), src, dst, n_ray,
+                           d, bg, M_inv, b, mode, counts, reinterpret_cast<unsigned int*>(emax));
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_brick_bins_fill_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                        const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
+                                        uint32_t* cursor, uint32_t* ids, int64_t n_ids, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "brick_bins_fill_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
+    if (n_ray == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(cursor && ids && src && dst && M_inv && b && n_ids >= 0, "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    const unsigned blocks = (unsigned)((n_ray + kBinRaysPerBlock - 1) / kBinRaysPerBlock);
+    if (nbk <= kBinMaxLdsBricks)
+        hipLaunchKernelGGL(k_bin_fill<true>, dim3(blocks), dim3(kBinThreads), (size_t)nbk * 4, as_stream(stream), src,
+                           dst, n_ray, d, bg, M_inv, b, mode, cursor, ids, n_ids);
+    else
+        hipLaunchKernelGGL(k_bin_fill<false>, dim3(blocks), dim3(kBinThreads), 0, as_stream(stream), src, dst, n_ray, d,
+                           bg, M_inv, b, mode, cursor, ids, n_ids);
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch, const float* src, const float* dst,
+                                                int64_t n_ray, const uint32_t* offsets, const uint32_t* ids,
+                                                const float* emax, const float* M_inv, const float* b, int64_t n_x,
+                                                int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
+                                                float* vmax, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "back_project_3d_brick_bins";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range", fn, (long long)batch);
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
+    CTLAB_REQUIRE(out && offsets && ids && emax && vmax && M_inv && b, "%s: null pointer", fn);
+    CTLAB_REQUIRE(n_ray == 0 || (sino && src && dst), "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    const int64_t V = n_x * n_y * n_z;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    st = fill_f32(vmax, batch, 0.0f, as_stream(stream));
+    if (st) return st;
+    if (n_ray > 0) {
+        const unsigned blocks = (unsigned)std::min<int64_t>((n_ray + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_absmax_all, dim3(blocks, (unsigned)batch), dim3(256), 0, as_stream(stream), sino, n_ray,
+                           reinterpret_cast<unsigned int*>(vmax));
+        st = check_launch(fn);
+        if (st) return st;
+    }
+    const int split = accumulate ? 1 : 2;
+    if (split > 1) {
+        st = fill_f32(out, batch * V, 0.0f, as_stream(stream));
+        if (st) return st;
+    }
+    dim3 grid((unsigned)(nbk * split), (unsigned)batch);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL(k_back_brick_list<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
+                           n_ray, V, d, bg, offsets, ids, emax, src, dst, M_inv, b, vmax, accumulate, split);
+    else
+        hipLaunchKernelGGL(k_back_brick_list<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
+                           n_ray, V, d, bg, offsets, ids, emax, src, dst, M_inv, b, vmax, accumulate, split);
+    return check_launch(fn);
+}

@@ -140,16 +140,22 @@ class ProjectorOps:
             opts = proj._options() if self.table is None else None
             self.det_shape = None if opts is None else opts.det_shape
             self.plan = None if opts is None else opts.plan
+            self.bins = None
             if self.table is None and self.plan is None:
-                # rays given without a layout: recognise it now, outside any graph capture
+                # rays given without a layout: recognise it now, outside any graph capture,
+                # else list the rays of each brick
                 layout = _C._layout_of(self.src, self.dst)
                 plan = _C._plan_of(self.src, self.dst, layout, self.minv, self.b, tuple(self.shape)) if layout else None
                 if plan is not None:
                     self.det_shape, self.plan = layout, plan
+                elif self.det_shape is None:
+                    self.bins = _C._bins_of(self.src, self.dst, self.minv, self.b, tuple(self.shape), self.mode)
             self.ws = None
             if self.plan is not None:
                 n_view = self.n_ray // (self.det_shape[0] * self.det_shape[1])
                 self.ws = torch.empty(n_view, device=self.device)
+            elif self.bins is not None:
+                self.ws = torch.empty(1, device=self.device)
         else:
             # the module's own untouched table runs on the fly with the table's rule
             self.table, sem, zr = proj._rule()
@@ -182,6 +188,9 @@ class ProjectorOps:
             if self.plan is not None:
                 return _C.back_project_3d_brick(y, self.src, self.dst, self.det_shape, self.plan, self.minv, self.b,
                                                 n_x, n_y, n_z, mode=self.mode, out=out, workspace=self.ws)
+            if self.bins is not None:
+                return _C.back_project_3d_brick_bins(y, self.src, self.dst, self.bins, self.minv, self.b, n_x, n_y, n_z,
+                                                     mode=self.mode, out=out, workspace=self.ws)
             return _C.back_project_3d_on_the_fly_cuda(y, self.src, self.dst, self.minv, self.b, n_x, n_y, n_z,
                                                       mode=self.mode, det_shape=self.det_shape, out=out)
         n_row, n_col = self.shape

@@ -255,6 +255,37 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
                                 int accumulate, float* out, float* view_max, ctlab_stream_t stream);
 
+/* Brick back projection of rays in any order (no reference counterpart: an
+ * alternative schedule of _C.back_project_3d_on_the_fly_cuda, bindings.cpp:101-103,
+ * for rays that are not a stack of flat-panel views).  A binning pass lists, per
+ * 32x32x8 brick, the rays that can have a segment in it (the rays passing within
+ * E + 0.05 voxel of the brick's cells, E the ray's midpoint error bound):
+ *   1. ctlab_brick_bins_count_3d writes counts[n_brick] (rays per brick) and
+ *      emax[1] (the largest E over rays with segments);
+ *   2. the caller forms offsets[n_brick + 1], the exclusive prefix sums of the
+ *      counts, and a copy of offsets[0..n_brick) as cursor;
+ *   3. ctlab_brick_bins_fill_3d writes the ray ids of brick k to
+ *      ids[offsets[k] .. offsets[k+1]) (in an order that depends on timing;
+ *      n_ids, the length of ids, bounds every write, and the back projection
+ *      skips ids of n_ray or more, so fill ids with 0xffffffff first).
+ * The lists depend only on the rays, grid, map and rule: build them once per ray
+ * set.  ctlab_back_project_3d_brick_bins then computes the same segments as
+ * ctlab_back_project_3d_otf brick by brick in LDS (64-bit fixed point, no global
+ * atomics, results independent of the order of the lists); vmax is caller
+ * workspace of batch floats.  n_ray must be below 2^32.  ctlab_brick_count_3d
+ * returns n_brick. */
+int64_t ctlab_brick_count_3d(int64_t n_x, int64_t n_y, int64_t n_z);
+int ctlab_brick_bins_count_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv, const float* b,
+                              int64_t n_x, int64_t n_y, int64_t n_z, int mode, uint32_t* counts, float* emax,
+                              ctlab_stream_t stream);
+int ctlab_brick_bins_fill_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv, const float* b,
+                             int64_t n_x, int64_t n_y, int64_t n_z, int mode, uint32_t* cursor, uint32_t* ids,
+                             int64_t n_ids, ctlab_stream_t stream);
+int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch, const float* src, const float* dst,
+                                     int64_t n_ray, const uint32_t* offsets, const uint32_t* ids, const float* emax,
+                                     const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
+                                     int mode, int accumulate, float* out, float* vmax, ctlab_stream_t stream);
+
 /* Largest batch * n_view of ctlab_back_project_3d_brick_noalloc (256 KiB of
  * per-view maxima, held by each device the library runs on until the process
  * ends). */
