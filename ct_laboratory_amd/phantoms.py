@@ -42,7 +42,12 @@ def _random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def ellipsoid_phantom_3d(n_x, n_y, n_z, voxel_size=1.0, n_ellipsoid=20, seed=1, background=0.0):
+def ellipsoid_phantom_3d(n_x, n_y, n_z, voxel_size=1.0, n_ellipsoid=20, seed=1, background=0.0, device=None):
+    """With ``device`` (a torch device) the same float32 operations run there
+    and a tensor on that device is returned (the ellipsoids' parameters still
+    come from the seeded numpy generator)."""
+    if device is not None:
+        return _ellipsoid_phantom_3d_torch(n_x, n_y, n_z, voxel_size, n_ellipsoid, seed, background, device)
     rng = np.random.default_rng(seed)
     x = _centres(n_x, voxel_size).astype(np.float32)[:, None, None]
     y = _centres(n_y, voxel_size).astype(np.float32)[None, :, None]
@@ -59,4 +64,28 @@ def ellipsoid_phantom_3d(n_x, n_y, n_z, voxel_size=1.0, n_ellipsoid=20, seed=1, 
             comp = R[0, k] * dx + R[1, k] * dy + R[2, k] * dz
             q += (comp / ax[k]) ** 2
         vol += mu * (q <= 1.0)
+    return vol
+
+
+def _ellipsoid_phantom_3d_torch(n_x, n_y, n_z, voxel_size, n_ellipsoid, seed, background, device):
+    import torch
+
+    rng = np.random.default_rng(seed)
+
+    def centres(n, shape):
+        return torch.from_numpy(_centres(n, voxel_size).astype(np.float32)).to(device).view(shape)
+
+    x, y, z = centres(n_x, (-1, 1, 1)), centres(n_y, (1, -1, 1)), centres(n_z, (1, 1, -1))
+    vol = torch.full((n_x, n_y, n_z), float(background), dtype=torch.float32, device=device)
+    for _ in range(n_ellipsoid):
+        c = rng.uniform(-100, 100, 3).astype(np.float32)
+        ax = rng.uniform(5, 100, 3).astype(np.float32)
+        R = _random_rotation(rng).astype(np.float32)
+        mu = np.float32(rng.uniform(0.005, 0.04))
+        dx, dy, dz = x - float(c[0]), y - float(c[1]), z - float(c[2])
+        q = torch.zeros((n_x, n_y, n_z), dtype=torch.float32, device=device)
+        for k in range(3):
+            comp = float(R[0, k]) * dx + float(R[1, k]) * dy + float(R[2, k]) * dz
+            q += (comp / float(ax[k])) ** 2
+        vol += float(mu) * (q <= 1.0)
     return vol

@@ -77,7 +77,7 @@ def build_problem(req: MAPRequest, device) -> None:
     if req.kind == "fan2d":
         x = torch.from_numpy(ellipse_phantom_2d(req.n, req.n, 1.0, seed=req.seed)).to(device)
     else:
-        x = torch.from_numpy(ellipsoid_phantom_3d(req.n, req.n, req.n, 1.0, seed=req.seed)).to(device)
+        x = ellipsoid_phantom_3d(req.n, req.n, req.n, 1.0, seed=req.seed, device=device)
     y = proj.forward_project(x)
     sigma = 0.01 * float(y.max())
     g = torch.Generator().manual_seed(10_000 + req.seed)

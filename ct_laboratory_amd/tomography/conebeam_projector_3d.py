@@ -19,7 +19,14 @@ __all__ = ["build_conebeam_3d_geometry", "ConeBeam3DProjector"]
 
 def build_conebeam_3d_geometry(n_view, det_nx, det_ny, sid, sdd, det_spacing=1.0, angle_offset=0.0,
                                volume_center_xyz=(0.0, 0.0, 0.0), device=None):
-    """Returns src, dst of shape [n_view * det_ny * det_nx, 3] (float32)."""
+    """Returns src, dst of shape [n_view * det_ny * det_nx, 3] (float32).
+
+    The per-view quantities (angles, their cosines and sines, source, detector
+    centre and in-plane direction: n_view values each) are computed on the CPU;
+    with ``device`` given, the expansion over the detector pixels (float32
+    multiplies and adds, correctly rounded on either processor) runs there, so
+    the rays equal the CPU-built ones bit for bit without a host copy of
+    n_view * det_ny * det_nx rays."""
     cx, cy, cz = volume_center_xyz
     ang = torch.linspace(0, 2 * math.pi - 2 * math.pi / n_view, n_view, dtype=torch.float32) + angle_offset
     c, s = torch.cos(ang), torch.sin(ang)
@@ -33,6 +40,9 @@ def build_conebeam_3d_geometry(n_view, det_nx, det_ny, sid, sdd, det_spacing=1.0
     pz = torch.zeros_like(px)
     u = (torch.arange(det_nx, dtype=torch.float32) - (det_nx - 1) / 2.0) * det_spacing
     v = (torch.arange(det_ny, dtype=torch.float32) - (det_ny - 1) / 2.0) * det_spacing
+    if device is not None:
+        u, v = u.to(device), v.to(device)
+        sx, sy, sz, dxc, dyc, dzc, px, py, pz = (t.to(device) for t in (sx, sy, sz, dxc, dyc, dzc, px, py, pz))
     U = u.unsqueeze(0).expand(det_ny, det_nx)
     V = v.unsqueeze(1).expand(det_ny, det_nx)
     sh = (n_view, 1, 1)
@@ -43,8 +53,6 @@ def build_conebeam_3d_geometry(n_view, det_nx, det_ny, sid, sdd, det_spacing=1.0
     src = torch.stack([sx.expand(-1, det_ny, det_nx), sy.expand(-1, det_ny, det_nx),
                        sz.expand(-1, det_ny, det_nx)], dim=-1).reshape(-1, 3)
     dst = torch.stack([cell_x, cell_y, cell_z], dim=-1).reshape(-1, 3)
-    if device is not None:
-        src, dst = src.to(device), dst.to(device)
     return src, dst
 
 
@@ -58,13 +66,13 @@ class ConeBeam3DProjector(CTProjector3DModule):
     def __init__(self, n_x, n_y, n_z, n_view, det_nx, det_ny, sid, sdd, det_spacing=1.0, voxel_size=1.0,
                  angle_offset=0.0, backend="cuda", device=None, precomputed_intersections=False, *,
                  semantics: str = "torch"):
-        src, dst = build_conebeam_3d_geometry(n_view=n_view, det_nx=det_nx, det_ny=det_ny, sid=sid, sdd=sdd,
-                                              det_spacing=det_spacing, angle_offset=angle_offset,
-                                              volume_center_xyz=(0.0, 0.0, 0.0))
-        M, b = standard_image_transform_3d(n_x=n_x, n_y=n_y, n_z=n_z, s_x=voxel_size, s_y=voxel_size,
-                                           s_z=voxel_size)
         if device is None:
             device = torch.device("cuda")
+        src, dst = build_conebeam_3d_geometry(n_view=n_view, det_nx=det_nx, det_ny=det_ny, sid=sid, sdd=sdd,
+                                              det_spacing=det_spacing, angle_offset=angle_offset,
+                                              volume_center_xyz=(0.0, 0.0, 0.0), device=device)
+        M, b = standard_image_transform_3d(n_x=n_x, n_y=n_y, n_z=n_z, s_x=voxel_size, s_y=voxel_size,
+                                           s_z=voxel_size)
         self.n_view, self.det_nx, self.det_ny = int(n_view), int(det_nx), int(det_ny)
         super().__init__(n_x=n_x, n_y=n_y, n_z=n_z, M=M.to(device), b=b.to(device), src=src.to(device),
                          dst=dst.to(device), backend=backend, device=device,

@@ -402,3 +402,16 @@ def test_copied_table_runs_on_the_fly(dev):
     x = img.clone().requires_grad_(True)
     proj(x).backward(y)
     assert rel_err(_np(x.grad), _np(own_back)) < 1e-6
+
+
+def test_conebeam_rays_built_on_the_device_equal_the_host_ones(dev):
+    """The cone-beam builder expands the per-view quantities (computed on the
+    CPU) over the detector on the device: the same float32 multiplies and adds,
+    so the rays equal the CPU-built ones bit for bit."""
+    from ct_laboratory_amd.tomography.conebeam_projector_3d import build_conebeam_3d_geometry
+
+    for args in ((37, 64, 48, 1000.0, 1500.0, 1.5, 0.3), (8, 384, 384, 600.0, 900.0, 2.0, 0.0)):
+        s_cpu, d_cpu = build_conebeam_3d_geometry(*args[:5], det_spacing=args[5], angle_offset=args[6])
+        s_dev, d_dev = build_conebeam_3d_geometry(*args[:5], det_spacing=args[5], angle_offset=args[6], device=dev)
+        assert s_dev.is_cuda and d_dev.is_cuda
+        assert torch.equal(s_dev.cpu(), s_cpu) and torch.equal(d_dev.cpu(), d_cpu)
