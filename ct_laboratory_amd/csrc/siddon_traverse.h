@@ -210,18 +210,21 @@ __host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const flo
 // Same as flat_voxel for finite coordinates with |c| < 2^22, in two integer-
 // exact steps per axis: RN(c + 1.5*2^23) lies in [2^23, 2^24) where the float
 // spacing is 1, so the addition itself rounds c half-to-even, and the bit
-// pattern minus that of 1.5*2^23 is the rounded integer.  The clipped
+// pattern minus that of 1.5*2^23 is the rounded integer.  kb[a] holds that
+// bit pattern plus the window's lower corner, so one subtraction gives the
+// window index.  The clipped
 // traversal only evaluates midpoints within a few voxels of the grid (and the
 // grid dimensions are below 2^24 with E < 1), so |c| stays far below 2^22.
 template <int ND>
-__host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w, const float* c, uint32_t* idx) {
+__host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w, const uint32_t* kb, const float* c,
+                                                            uint32_t* idx) {
     const float kMagic = 12582912.0f;  // 1.5 * 2^23
     uint32_t r[ND];
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const float f = c[a] + kMagic;
-        r[a] = __builtin_bit_cast(uint32_t, f) - (0x4b400000u + (uint32_t)w.lo[a]);
+        r[a] = __builtin_bit_cast(uint32_t, f) - kb[a];
         ok = ok && (r[a] < (uint32_t)w.ext[a]);
     }
     if (ND == 3) {
@@ -558,6 +561,17 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
     // form.
     MidIndex<ND, DIAG> mi;
     mi.init(g);
+    // bit pattern of 1.5 * 2^23 + lo per axis, read from the first active lane:
+    // every caller passes one window per wave (the whole grid, a block's tile
+    // or brick), so it sits in scalar registers
+    uint32_t kb[ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        kb[a] = 0x4b400000u + (uint32_t)win.lo[a];
+#if defined(__HIP_DEVICE_COMPILE__)
+        kb[a] = __builtin_amdgcn_readfirstlane(kb[a]);
+#endif
+    }
     auto walk = [&](auto exh, float t_stop) {
         constexpr bool EXH = decltype(exh)::value;
         float t0;
@@ -602,7 +616,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
             }
             t0 = t1;
             uint32_t idx;
-            const bool in = flat_voxel_bounded<ND>(win, c, &idx);
+            const bool in = flat_voxel_bounded<ND>(win, kb, c, &idx);
             if (in && len > 0.0f) visit(idx, len);
         };
         // PAIR: two segments per trip with the end-point registers swapping
