@@ -255,6 +255,143 @@ def generic_bench(a, dev) -> int:
     return 0
 
 
+# The reference's projector timing benchmark (benchmark/projector_timing_benchmark.py)
+# runs a calibrated 80-source, 3-rotation StaticCT ring that is not available here.
+# The synthetic ring below has its published shape (benchmark/README.md:36-40): 80
+# sources x 3 gantry rotations = 240 views, each lighting the 16 farthest of 48
+# inward-facing modules of 48 x 16 pixels (12,288 rays per view, 2,949,120 rays);
+# radii and pixel pitch are those of the reference's StaticCT example
+# (python_examples/test_staticct_projector_3d.py:43-48).  Rotation r turns the
+# gantry by r/3 of the source pitch.
+RING = dict(n_source=80, n_rotation=3, source_radius=380.0, n_module=48, module_radius=366.7, modules_per_source=16,
+            det_nx=48, det_ny=16, det_sx=1.0, det_sy=2.0)
+# res: (n_z, voxel pitch mm); 400 mm transaxial FOV, +5 mm z shift
+# (projector_timing_benchmark.py:43-51, 106-118)
+RING_GRIDS = {64: (8, 6.4), 128: (16, 3.2), 256: (32, 1.6), 512: (64, 0.8)}
+RING_B_SHIFT_Z = 5.0
+# H100 80GB HBM3, res512, CUDA kernels with uint16 tables: forward / back ms
+# (benchmark/README.md:102)
+RING_H100_U16_MS = (46.2, 36.1)
+
+
+def staticct_ring_rays(dev):
+    """src, dst [2,949,120, 3] of the synthetic 240-view ring, in the StaticCT
+    builders' order (frame, module, u, v)."""
+    import math
+
+    import torch
+
+    from ct_laboratory_amd.tomography.staticct_projector_3d import (build_static_3d_geometry,
+                                                                    build_uniform_static_3d_geometry)
+
+    r = RING
+    sp, mc, mo, mask = build_uniform_static_3d_geometry(r["n_source"], r["source_radius"], 0.0, r["n_module"],
+                                                        r["module_radius"], 0.0, r["modules_per_source"])
+    n_frame = r["n_source"] * r["n_rotation"]
+    Mg = torch.zeros(n_frame, 3, 3)
+    active = torch.zeros(n_frame, r["n_source"], dtype=torch.bool)
+    for f in range(n_frame):
+        ang = (f // r["n_source"]) * 2.0 * math.pi / (r["n_source"] * r["n_rotation"])
+        Mg[f] = torch.tensor([[math.cos(ang), -math.sin(ang), 0.0], [math.sin(ang), math.cos(ang), 0.0],
+                              [0.0, 0.0, 1.0]])
+        active[f, f % r["n_source"]] = True
+    return build_static_3d_geometry(sp, mc, mo, r["det_nx"], r["det_ny"], r["det_sx"], r["det_sy"], mask, Mg,
+                                    torch.zeros(n_frame, 3), active, device=dev)
+
+
+def ring_grid(res: int, dev):
+    """(M, b, n_z) of the 400 mm FOV grid (projector_timing_benchmark.py:106-118)."""
+    import torch
+
+    nz, sp = RING_GRIDS[res]
+    M = torch.eye(3, device=dev) * sp
+    b = torch.tensor([-(res - 1) / 2.0 * sp, -(res - 1) / 2.0 * sp, -(nz - 1) / 2.0 * sp + RING_B_SHIFT_Z], device=dev)
+    return M, b, nz
+
+
+def staticct_bench(a, dev) -> int:
+    """The reference's projector timing benchmark on the synthetic ring: for each
+    grid, CTProjector3DModule built as that benchmark builds it (uint16 tables
+    from precompute_tvals_stitched in chunks of 32,768 rays, float tables, or
+    none: on the fly), plus the on-the-fly module told the rays' module layout
+    (48 x 16 pixels per source and module); forward_project of a randn volume
+    and back_project of its sinogram, median wall ms over the repeats with a
+    synchronize either side of each call, as the reference times them.  Prints
+    one JSON line."""
+    import torch
+
+    from ct_laboratory_amd.tomography.ct_projector_3d_module import CTProjector3DModule, precompute_tvals_stitched
+
+    src, dst = staticct_ring_rays(dev)
+    n_ray = int(src.shape[0])
+    print(f"[staticct] ring built: {n_ray} rays", file=sys.stderr, flush=True)
+    repeats = max(3, a.steps)
+
+    def median_ms(fn):
+        vals, out = [], None
+        for _ in range(repeats + 1):  # the first call is a warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            vals.append(1e3 * (time.perf_counter() - t0))
+        return float(np.median(vals[1:])), out
+
+    rows = []
+    for res in a.ring_res:
+        M, b, nz = ring_grid(res, dev)
+        vol = torch.randn(res, res, nz, device=dev, generator=torch.Generator(device=dev).manual_seed(res))
+        ref = {}
+        for label in ("uint16", "float", "none", "none+layout"):
+            print(f"[staticct] res{res} {label}", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
+            torch.cuda.reset_peak_memory_stats(dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if label in ("uint16", "float"):
+                comp = label == "uint16"
+                tv = precompute_tvals_stitched(res, res, nz, M.cpu(), b.cpu(), src.cpu(), dst.cpu(), chunk_size=32768,
+                                               device=dev, verbose=False, use_compression=comp)
+                proj = CTProjector3DModule(res, res, nz, M, b, src, dst, device=dev, tvals=tv, use_compression=comp)
+                del tv
+            else:
+                det = (RING["det_nx"], RING["det_ny"]) if label == "none+layout" else None
+                proj = CTProjector3DModule(res, res, nz, M, b, src, dst, device=dev, det_shape=det)
+            torch.cuda.synchronize()
+            build_ms = 1e3 * (time.perf_counter() - t0)
+            fwd_ms, sino = median_ms(lambda: proj.forward_project(vol))
+            back_ms, bk = median_ms(lambda: proj.back_project(sino))
+            lhs = float(torch.dot(sino.reshape(-1).double(), sino.reshape(-1).double()))
+            rhs = float(torch.dot(vol.reshape(-1).double(), bk.reshape(-1).double()))
+            row = {"res": res, "n_z": nz, "voxel_mm": RING_GRIDS[res][1], "tvals": label, "build_ms": build_ms,
+                   "forward_ms": fwd_ms, "back_ms": back_ms,
+                   "pair_rays_per_s": n_ray / (1e-3 * (fwd_ms + back_ms)),
+                   "peak_gpu_gb": torch.cuda.max_memory_allocated(dev) / 1e9, "adjoint_rel_err": abs(lhs - rhs) / lhs}
+            if getattr(proj, "_runs", None) is not None:
+                row["table_runs"] = proj._runs.n_runs
+                row["table_units"] = proj._runs.n_units
+            if label in ("uint16", "float", "none"):
+                ref[label] = sino.reshape(-1)
+            if label != "uint16" and "uint16" in ref:
+                d = sino.reshape(-1) - ref["uint16"]
+                row["forward_rel_diff_vs_uint16"] = float(d.norm() / ref["uint16"].norm())
+            if label == "none+layout":
+                row["forward_equal_to_none"] = bool(torch.equal(sino.reshape(-1), ref["none"]))
+            if res == 512 and label == "uint16":
+                row["h100_forward_back_ms"] = list(RING_H100_U16_MS)
+                row["speedup_vs_h100"] = [RING_H100_U16_MS[0] / fwd_ms, RING_H100_U16_MS[1] / back_ms]
+            print(f"[staticct] {json.dumps(row)}", file=sys.stderr, flush=True)
+            rows.append(row)
+            del proj, sino, bk
+        del ref, vol
+    res = {"metric": "StaticCT ring (240 views x 12,288 rays) CTProjector3DModule forward / back ms, median",
+           "workload": "synthetic 80-source x 3-rotation ring, 16 of 48 modules of 48x16 px per view, "
+                       "2,949,120 rays; 400 mm FOV grids with +5 mm z shift; randn volume",
+           "n_ray": n_ray, "repeats": repeats, "rows": rows}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def batched_bench(a, dev) -> int:
     """64 independent 2D scans (sizes 128^2-512^2, 90-720 views, half fan beam, half
     StaticCT ring; scan i seeded by default_rng(100 + i)): forward + back of all
@@ -548,6 +685,10 @@ def main(argv=None) -> int:
                     help="the 256^3 cone-beam workload through the _C calls as the reference's Python makes them")
     ap.add_argument("--generic", action="store_true",
                     help="the 256^3 workload's rays in random order: back projection over brick lists vs atomics")
+    ap.add_argument("--staticct", action="store_true",
+                    help="the reference's projector timing benchmark on a synthetic 240-view StaticCT ring "
+                         "(uint16 / float tables / on the fly through CTProjector3DModule)")
+    ap.add_argument("--ring-res", type=int, nargs="+", default=[256, 512], choices=sorted(RING_GRIDS))
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven 2D scans: one persistent launch (autotuned) vs 64 sequential launches")
     ap.add_argument("--map", action="store_true",
@@ -580,6 +721,8 @@ def main(argv=None) -> int:
         return batched_bench(a, dev)
     if a.generic:
         return generic_bench(a, dev)
+    if a.staticct:
+        return staticct_bench(a, dev)
     if a.fan2d:
         return fan2d_bench(a, dev)
     if a.refcalls:

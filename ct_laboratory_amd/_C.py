@@ -46,6 +46,7 @@ import threading
 import weakref
 from collections import OrderedDict
 
+import numpy as np
 import torch
 
 from . import _native as N
@@ -364,6 +365,10 @@ def back_project_3d_cuda(sino, tvals, src, dst, M_inv, b, n_x, n_y, n_z, *, out=
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
+    runs = _truns_of(tvals, src, dst, M_inv, b, (n_x, n_y, n_z))
+    if runs is not None:
+        return back_project_3d_table_brick(sino, tvals, src, dst, runs, M_inv, b, n_x, n_y, n_z, out=out,
+                                           accumulate=accumulate)
     acc = bool(accumulate) and out is not None
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
@@ -431,11 +436,133 @@ def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, 
     _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
     K = _u16(name, t_uint16, t_start, t_scale, n_ray)
     _affine(name, M_inv, b, 3)
+    table = (t_uint16, t_start, t_scale)
+    runs = _truns_of(table, src, dst, M_inv, b, (n_x, n_y, n_z))
+    if runs is not None:
+        return back_project_3d_table_brick(sino, table, src, dst, runs, M_inv, b, n_x, n_y, n_z, out=out)
     out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_u16", N.ptr(sino), batch, N.ptr(t_uint16), N.ptr(t_start), N.ptr(t_scale), K,
                N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), 0, N.ptr(out),
                N.stream_of(sino))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Table back projection in LDS bricks (run lists of a crossing table)
+# ---------------------------------------------------------------------------
+
+TABLE_UNIT_RUNS = 16384  # runs per work unit of the table brick back projection
+
+
+class TableRuns:
+    """Run lists of a crossing table: runs int32 [n_runs, 4] holding the uint32
+    records (ray, first segment, segment count, k before the first segment),
+    grouped by brick; units int64 [n_units, 2] holding the (brick, n, first)
+    records, largest first; and the grid, table width and ray count."""
+
+    def __init__(self, runs, n_runs, units, shape, K, n_ray, kind, inputs):
+        self.runs, self.n_runs, self.units = runs, int(n_runs), units
+        self.shape, self.K, self.n_ray, self.kind = tuple(int(x) for x in shape), int(K), int(n_ray), kind
+        self._inputs = inputs
+
+    @property
+    def n_units(self) -> int:
+        return int(self.units.shape[0])
+
+    def matches(self, table, src, dst, M_inv, b) -> bool:
+        """Whether these are the very table, ray and map tensors (unwritten
+        since) that the lists were built from."""
+        return _table_inputs(table, src, dst, M_inv, b) == self._inputs
+
+
+def _table_ptrs(name, table, n_ray):
+    """(tvals, u16, start, scale, K, kind) pointers of a float table or a
+    (uint16, start, scale) tuple."""
+    if isinstance(table, (tuple, list)):
+        t16, ts, tsc = table
+        K = _u16(name, t16, ts, tsc, n_ray)
+        return 0, N.ptr(t16), N.ptr(ts), N.ptr(tsc), K, "u16"
+    K = _tvals(name, table, n_ray)
+    return N.ptr(table), 0, 0, 0, K, "f32"
+
+
+def _table_inputs(table, src, dst, M_inv, b):
+    parts = table if isinstance(table, (tuple, list)) else (table,)
+    return (tuple((id(p), p.data_ptr(), p._version) for p in parts), _ray_key(src, dst), id(src), id(dst),
+            _map_values(M_inv), _map_values(b))
+
+
+def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_RUNS):
+    """Run lists of ``table`` (float [n_ray, K], or the (uint16, start, scale)
+    tuple of the CUDA decode) for these rays, map and grid: two passes over the
+    table and one synchronisation (the brick counts come to the host to size
+    the run list and cut the work units)."""
+    name = "table_runs"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 3)
+    _affine(name, M_inv, b, 3)
+    _require(n_ray < 2 ** 32, f"{name}: at most 2^32 - 1 rays")
+    tv, t16, ts, tsc, K, kind = _table_ptrs(name, table, n_ray)
+    nx, ny, nz = (int(x) for x in shape)
+    L = N.load()
+    nbk = int(L.ctlab_table_brick_count(nx, ny, nz))
+    _require(nbk > 0, f"{name}: bad grid")
+    dev = src.device
+    counts = torch.empty((nbk,), device=dev, dtype=torch.int32)
+    with torch.cuda.device(dev):
+        N.call("ctlab_table_runs_count_3d", nx, ny, nz, tv, t16, ts, tsc, K, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), N.ptr(counts), N.stream_of(src))
+        c = counts.cpu().numpy().astype(np.int64) & 0xFFFFFFFF
+        offsets_h = np.zeros(nbk + 1, dtype=np.int64)
+        np.cumsum(c, out=offsets_h[1:])
+        total = int(offsets_h[-1])
+        offsets = torch.from_numpy(offsets_h).to(dev)
+        runs = torch.empty((max(total, 1), 4), device=dev, dtype=torch.int32)
+        cursor = torch.empty((nbk,), device=dev, dtype=torch.int32)
+        N.call("ctlab_table_runs_fill_3d", nx, ny, nz, tv, t16, ts, tsc, K, N.ptr(src), N.ptr(dst), n_ray,
+               N.ptr(M_inv), N.ptr(b), N.ptr(offsets), N.ptr(cursor), N.ptr(runs), total, N.stream_of(src))
+    # work units: slices of at most unit_runs runs of one brick, largest first
+    u = max(1, int(unit_runs))
+    bricks = np.nonzero(c)[0]
+    per = (c[bricks] + u - 1) // u
+    ub = np.repeat(bricks, per)
+    k = np.arange(len(ub)) - np.repeat(np.cumsum(per) - per, per)
+    first = offsets_h[ub] + k * u
+    n = np.minimum(u, c[ub] - k * u)
+    order = np.argsort(-n, kind="stable")
+    units_h = np.empty((len(ub), 2), dtype=np.int64)
+    units_h[:, 0] = ub[order] | (n[order] << 32)
+    units_h[:, 1] = first[order]
+    units = torch.from_numpy(units_h).to(dev)
+    result = TableRuns(runs, total, units, shape, K, n_ray, kind, _table_inputs(table, src, dst, M_inv, b))
+    if not _capturing():
+        _remember_truns(table, src, dst, M_inv, b, shape, result)
+    return result
+
+
+def back_project_3d_table_brick(sino, table, src, dst, runs, M_inv, b, n_x, n_y, n_z, *, out=None,
+                                accumulate: bool = False):
+    """[B, n_x, n_y, n_z] back projection from a crossing table over its run
+    lists (table_runs for this table, rays, map and grid): the segments and
+    values of back_project_3d_cuda / back_project_3d_compressed_cuda, summed in
+    LDS bricks instead of one global atomic per segment."""
+    name = "back_project_3d_table_brick"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    tv, t16, ts, tsc, K, kind = _table_ptrs(name, table, n_ray)
+    _affine(name, M_inv, b, 3)
+    _require(isinstance(runs, TableRuns) and runs.shape == (int(n_x), int(n_y), int(n_z)) and runs.n_ray == n_ray
+             and runs.K == K and runs.kind == kind and runs.matches(table, src, dst, M_inv, b),
+             f"{name}: the run lists were built for another table, grid, rays or map, or those tensors were "
+             "written since")
+    acc = bool(accumulate) and out is not None
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_table_brick", N.ptr(sino), batch, tv, t16, ts, tsc, K, N.ptr(src), N.ptr(dst),
+               n_ray, N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), N.ptr(runs.runs), runs.n_runs,
+               N.ptr(runs.units), runs.n_units, int(acc), N.ptr(out), N.stream_of(sino))
     return out
 
 
@@ -815,6 +942,53 @@ def _bins_of(src, dst, M_inv, b, shape, mode):
         while len(_BINS) > _BINS_CACHE_SIZE:
             _BINS.popitem(last=False)
     return bins
+
+
+_TRUNS: OrderedDict = OrderedDict()  # (table key, ray key, grid, M_inv and b values) -> (refs, TableRuns)
+_TRUNS_CACHE_SIZE = 2
+_MIN_TABLE_ENTRIES = 1 << 22  # below this the per-segment atomic kernels are as fast
+
+
+def _table_key(table):
+    parts = table if isinstance(table, (tuple, list)) else (table,)
+    return tuple((p.data_ptr(), p._version, tuple(p.shape)) for p in parts), tuple(weakref.ref(p) for p in parts), parts
+
+
+def _truns_key(table, src, dst, M_inv, b, shape):
+    tkey, trefs, parts = _table_key(table)
+    return (tkey, _ray_key(src, dst), tuple(int(x) for x in shape), _map_values(M_inv), _map_values(b)), trefs, parts
+
+
+def _remember_truns(table, src, dst, M_inv, b, shape, runs):
+    """Files run lists built by table_runs under their table, rays, map and grid."""
+    key, trefs, _ = _truns_key(table, src, dst, M_inv, b, shape)
+    with _CACHE_LOCK:
+        for k in [k for k, v in _TRUNS.items() if v[0]() is None or any(r() is None for r in v[2])]:
+            del _TRUNS[k]
+        _TRUNS[key] = (weakref.ref(src), weakref.ref(dst), trefs, runs)
+        _TRUNS.move_to_end(key)
+        while len(_TRUNS) > _TRUNS_CACHE_SIZE:
+            _TRUNS.popitem(last=False)
+
+
+def _truns_of(table, src, dst, M_inv, b, shape):
+    """Run lists of a crossing table (the last ones table_runs built for it, or
+    new ones), kept per table, ray set, map values and grid while those tensors
+    are alive and unwritten; None while a graph is being captured (building
+    them synchronises) or for small tables."""
+    if not AUTO_LAYOUT or _capturing():
+        return None
+    t0 = table[0] if isinstance(table, (tuple, list)) else table
+    if t0.numel() < _MIN_TABLE_ENTRIES:
+        return None
+    key, _, parts = _truns_key(table, src, dst, M_inv, b, shape)
+    with _CACHE_LOCK:
+        e = _TRUNS.get(key)
+        if (e is not None and _alive(e[0], src) and _alive(e[1], dst)
+                and all(_alive(r, p) for r, p in zip(e[2], parts))):
+            _TRUNS.move_to_end(key)
+            return e[3]
+    return table_runs(table, src, dst, M_inv, b, shape)
 
 
 def _plan_of(src, dst, layout, M_inv, b, shape):

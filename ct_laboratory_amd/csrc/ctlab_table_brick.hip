@@ -1,0 +1,354 @@
+// ctlab_table_brick.hip -- back projection over precomputed crossing tables
+// (float, or uint16 with the CUDA decode) that accumulates in LDS bricks
+// instead of adding every segment to the volume with a global atomic
+// (the reference's consumers, src/ct_projector_3d.cu:215-269 and 325-376, add
+// each segment with atomicAdd).
+//
+// A table fixes each ray's segments and their voxels.  The volume is cut into
+// bricks of 32 x 32 x 16 voxels (64 KB of floats).  A geometry-only pass
+// splits every ray's row into runs: maximal stretches of consecutive visited
+// segments whose voxels lie in one brick (a ray that comes back to a brick
+// starts another run).  Runs are grouped by brick; a work unit is a slice of
+// one brick's runs.  The back projection gives each unit one block, which
+// walks its runs' segments exactly as the table walkers do (same entries, same
+// arithmetic, same skip rules), adds val * len into the LDS brick with LDS
+// atomics, and then adds the brick's nonzero voxels to the volume with one
+// global atomic each.  Every segment the reference's consumer adds is added
+// once, to the same voxel, with the same value; only the grouping of the
+// float sums differs (the reference's atomic order is unspecified as well).
+//
+//   count   runs per brick (global atomics on the brick counters)
+//   fill    run records placed at offsets[brick] + a per-brick cursor
+//   back    one block per (unit, batch item)
+#include "ctlab_common.h"
+#include "siddon_traverse.h"
+#include "table_walk.h"
+
+namespace ctlab {
+
+constexpr int kTbX = CTLAB_TABLE_BRICK_X, kTbY = CTLAB_TABLE_BRICK_Y, kTbZ = CTLAB_TABLE_BRICK_Z;
+constexpr int kTbVox = kTbX * kTbY * kTbZ;
+constexpr int kTbBlock = 512;  // 8 waves per block, two blocks (128 KB of LDS) per CU
+
+static_assert(sizeof(ctlab_table_run_t) == 16, "run record is 16 bytes");
+static_assert(sizeof(ctlab_table_unit_t) == 16, "unit record is 16 bytes");
+
+struct Dims3 {
+    int n[3];
+};
+
+struct BrickGrid {
+    int nb[3];  // bricks per axis
+    __host__ __device__ __forceinline__ uint32_t id(const int32_t* r) const {
+        return ((uint32_t)(r[0] / kTbX) * (uint32_t)nb[1] + (uint32_t)(r[1] / kTbY)) * (uint32_t)nb[2] +
+               (uint32_t)(r[2] / kTbZ);
+    }
+};
+
+// Pass over every ray's row: FILL = false counts the runs of each brick,
+// FILL = true writes each run's record at offsets[brick] + cursor[brick]++.
+template <bool U16, bool FILL>
+__global__ __launch_bounds__(kBlock) void k_table_runs(Dims3 dims, BrickGrid bg, const float* __restrict__ tvals,
+                                                       const uint16_t* __restrict__ u16,
+                                                       const float* __restrict__ start,
+                                                       const float* __restrict__ scale, int64_t K,
+                                                       const float* __restrict__ src, const float* __restrict__ dst,
+                                                       int64_t n_ray, const float* __restrict__ minv,
+                                                       const float* __restrict__ b, uint32_t* __restrict__ counts,
+                                                       const int64_t* __restrict__ offsets,
+                                                       ctlab_table_run_t* __restrict__ runs, int64_t n_runs) {
+    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t r = r0 + threadIdx.x;
+    const bool active = r < n_ray;
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    float s[3] = {}, d[3] = {};
+    if (active) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s[a] = src[r * 3 + a];
+            d[a] = dst[r * 3 + a];
+        }
+    }
+    uint32_t cur = 0xffffffffu, k0 = 0;
+    int64_t seg0 = 0, last = 0;
+    auto close = [&]() {
+        if (!FILL) {
+            atomicAdd(counts + cur, 1u);
+        } else {
+            const int64_t pos = offsets[cur] + (int64_t)atomicAdd(counts + cur, 1u);
+            if (pos < n_runs) {
+                ctlab_table_run_t rec;
+                rec.ray = (uint32_t)r;
+                rec.seg0 = (uint32_t)seg0;
+                rec.nseg = (uint32_t)(last - seg0 + 1);
+                rec.k0 = k0;
+                runs[pos] = rec;
+            }
+        }
+    };
+    auto visit = [&](int64_t j, uint32_t kp, const float* c, float) {
+        int32_t v[3];
+        if (!round_voxel<3>(g, c, v)) return;
+        const uint32_t id = bg.id(v);
+        if (id != cur) {
+            if (cur != 0xffffffffu) close();
+            cur = id;
+            seg0 = j;
+            k0 = kp;
+        }
+        last = j;
+    };
+    if constexpr (U16) {
+        __shared__ RowStage<uint16_t, kStageU16> st;
+        const float st0 = active ? start[r] : 0.0f, sc = active ? scale[r] : 0.0f;
+        if (g.diag)
+            walk_u16<true>(g, s, d, u16, K, r0, n_ray, active, st0, sc, st, visit);
+        else
+            walk_u16<false>(g, s, d, u16, K, r0, n_ray, active, st0, sc, st, visit);
+    } else {
+        __shared__ RowStage<float, kStageF32> st;
+        if (g.diag)
+            walk_table<3, true>(g, s, d, tvals, K, r0, n_ray, active, st, visit);
+        else
+            walk_table<3, false>(g, s, d, tvals, K, r0, n_ray, active, st, visit);
+    }
+    if (active && cur != 0xffffffffu) close();
+}
+
+// The runs of one unit added into the LDS brick acc (origin x0, y0, z0).
+template <bool U16, bool DIAG>
+__device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, int x0, int y0, int z0,
+                                                 const float* __restrict__ sino, const float* __restrict__ tvals,
+                                                 const uint16_t* __restrict__ u16, const float* __restrict__ start,
+                                                 const float* __restrict__ scale, int64_t K,
+                                                 const float* __restrict__ src, const float* __restrict__ dst,
+                                                 int64_t n_ray, const ctlab_table_run_t* __restrict__ runs,
+                                                 int64_t n_runs, const ctlab_table_unit_t& u, int64_t bi) {
+    MidIndex<3, DIAG> mi;
+    mi.init(g);
+    const float* __restrict__ sn = sino + bi * n_ray;
+    auto add = [&](const float* c, float w) {
+        int32_t v[3];
+        if (!round_voxel<3>(g, c, v)) return;
+        const uint32_t lx = (uint32_t)(v[0] - x0), ly = (uint32_t)(v[1] - y0), lz = (uint32_t)(v[2] - z0);
+        if (lx < (uint32_t)kTbX && ly < (uint32_t)kTbY && lz < (uint32_t)kTbZ)
+            atomicAdd(&acc[(lx * kTbY + ly) * kTbZ + lz], w);
+    };
+    for (int64_t q = threadIdx.x; q < (int64_t)u.n; q += kTbBlock) {
+        const int64_t pos = u.first + q;
+        if (pos >= n_runs) break;
+        const ctlab_table_run_t rr = runs[pos];
+        const int64_t ray = rr.ray;
+        const int64_t j0 = rr.seg0, j1 = (int64_t)rr.seg0 + rr.nseg;
+        if (ray >= n_ray || j0 < 1 || j1 > K) continue;
+        const float val = sn[ray];
+        if (val == 0.0f) continue;
+        float s[3], vw[3], p0[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s[a] = src[ray * 3 + a];
+            vw[a] = dst[ray * 3 + a] - s[a];
+        }
+        if (U16) {
+            const float st0 = start[ray], sc = scale[ray];
+            // t of entry j0-1: start for entry 0, else start + k * scale (3d.cu:128-129)
+            const float t0 = (j0 == 1) ? st0 : __builtin_fmaf((float)rr.k0, sc, st0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
+            const uint16_t* __restrict__ row = u16 + ray * K;
+            uint32_t ks = rr.k0;
+            for (int64_t j = j0; j < j1; ++j) {
+                ks += (uint32_t)row[j];
+                const float t1 = __builtin_fmaf((float)ks, sc, st0);
+                float c[3];
+                const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
+                if (!(len >= 1e-9f)) continue;  // 3d.cu:137
+                add(c, val * len);
+            }
+        } else {
+            const float* __restrict__ row = tvals + ray * K;
+            float t0 = row[j0 - 1];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
+            for (int64_t j = j0; j < j1; ++j) {
+                const float t1 = row[j];
+                if (isinf(t0) || isinf(t1)) {  // end of a sorted row; a finite value restarts the walk
+                    t0 = t1;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
+                    continue;
+                }
+                float c[3];
+                const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
+                t0 = t1;
+                if (len > 0.0f) add(c, val * len);
+            }
+        }
+    }
+}
+
+template <bool U16>
+__global__ __launch_bounds__(kTbBlock) void k_back_table_brick(
+    const float* __restrict__ sino, Dims3 dims, BrickGrid bg, const float* __restrict__ tvals,
+    const uint16_t* __restrict__ u16, const float* __restrict__ start, const float* __restrict__ scale, int64_t K,
+    const float* __restrict__ src, const float* __restrict__ dst, int64_t n_ray, const float* __restrict__ minv,
+    const float* __restrict__ b, const ctlab_table_run_t* __restrict__ runs, int64_t n_runs,
+    const ctlab_table_unit_t* __restrict__ units, int64_t vol_elems, float* __restrict__ out) {
+    __shared__ float acc[kTbVox];
+    const ctlab_table_unit_t u = units[blockIdx.x];
+    const int64_t bi = blockIdx.y;
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const uint32_t n_brick = (uint32_t)bg.nb[0] * (uint32_t)bg.nb[1] * (uint32_t)bg.nb[2];
+    if (u.brick >= n_brick || !g.finite) return;  // uniform over the block
+    const int bz = (int)(u.brick % (uint32_t)bg.nb[2]);
+    const int by = (int)((u.brick / (uint32_t)bg.nb[2]) % (uint32_t)bg.nb[1]);
+    const int bx = (int)(u.brick / ((uint32_t)bg.nb[2] * (uint32_t)bg.nb[1]));
+    const int x0 = bx * kTbX, y0 = by * kTbY, z0 = bz * kTbZ;
+    for (int i = threadIdx.x; i < kTbVox; i += kTbBlock) acc[i] = 0.0f;
+    __syncthreads();
+    if (g.diag)
+        table_brick_runs<U16, true>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
+                                    n_runs, u, bi);
+    else
+        table_brick_runs<U16, false>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
+                                     n_runs, u, bi);
+    __syncthreads();
+    float* __restrict__ o = out + bi * vol_elems;
+    for (int i = threadIdx.x; i < kTbVox; i += kTbBlock) {
+        const float w = acc[i];
+        if (w == 0.0f) continue;
+        const int x = x0 + i / (kTbY * kTbZ), y = y0 + (i / kTbZ) % kTbY, z = z0 + i % kTbZ;
+        if (x < dims.n[0] && y < dims.n[1] && z < dims.n[2])
+            unsafeAtomicAdd(o + ((int64_t)x * dims.n[1] + y) * dims.n[2] + z, w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static BrickGrid brick_grid(const int64_t* n) {
+    BrickGrid bg;
+    bg.nb[0] = (int)((n[0] + kTbX - 1) / kTbX);
+    bg.nb[1] = (int)((n[1] + kTbY - 1) / kTbY);
+    bg.nb[2] = (int)((n[2] + kTbZ - 1) / kTbZ);
+    return bg;
+}
+
+struct TableArgs {
+    const float* tvals;
+    const uint16_t* u16;
+    const float* start;
+    const float* scale;
+    int64_t K;
+};
+
+static int check_table_geometry(const char* fn, const int64_t* n, const TableArgs& t, const float* src,
+                                const float* dst, int64_t n_ray, const float* minv, const float* b) {
+    for (int a = 0; a < 3; ++a)
+        CTLAB_REQUIRE(n[a] >= 1 && n[a] <= 0x7fffffffLL, "%s: grid dimension %d is %lld (must be >= 1)", fn, a,
+                      (long long)n[a]);
+    CTLAB_REQUIRE(n[0] * n[1] * n[2] <= kMaxVoxels, "%s: grid has more than 2^31-1 voxels", fn);
+    int st = check_index_limits(fn, n, 3);
+    if (st) return st;
+    CTLAB_REQUIRE((t.tvals != nullptr) != (t.u16 != nullptr), "%s: pass exactly one of tvals and u16", fn);
+    CTLAB_REQUIRE(!t.u16 || (t.start && t.scale), "%s: a uint16 table needs start and scale", fn);
+    CTLAB_REQUIRE(t.K >= 0 && t.K <= 0x7fffffffLL, "%s: K %lld out of range", fn, (long long)t.K);
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range [0, 2^32)", fn,
+                  (long long)n_ray);
+    CTLAB_REQUIRE(grid_for(n_ray) <= kMaxGridX, "%s: too many rays", fn);
+    if (n_ray > 0) CTLAB_REQUIRE(src && dst && minv && b, "%s: null pointer", fn);
+    return CTLAB_OK;
+}
+
+template <bool FILL>
+static int table_runs(const char* fn, const int64_t* n, const TableArgs& t, const float* src, const float* dst,
+                      int64_t n_ray, const float* minv, const float* b, uint32_t* counts, const int64_t* offsets,
+                      ctlab_table_run_t* runs, int64_t n_runs, hipStream_t stream) {
+    int st = check_table_geometry(fn, n, t, src, dst, n_ray, minv, b);
+    if (st) return st;
+    const BrickGrid bg = brick_grid(n);
+    const int64_t n_brick = (int64_t)bg.nb[0] * bg.nb[1] * bg.nb[2];
+    CTLAB_REQUIRE(counts, "%s: counts is null", fn);
+    if (FILL) CTLAB_REQUIRE(offsets && (n_runs == 0 || runs) && n_runs >= 0, "%s: null run buffers", fn);
+    // zero bits: the counters are cleared by the fill kernel (never hipMemsetAsync)
+    st = fill_f32(reinterpret_cast<float*>(counts), n_brick, 0.0f, stream);
+    if (st) return st;
+    if (n_ray == 0 || t.K < 2) return CTLAB_OK;
+    Dims3 dims;
+    for (int a = 0; a < 3; ++a) dims.n[a] = (int)n[a];
+    const dim3 grid(grid_for(n_ray)), block(kBlock);
+    if (t.u16)
+        hipLaunchKernelGGL((k_table_runs<true, FILL>), grid, block, 0, stream, dims, bg, t.tvals, t.u16, t.start,
+                           t.scale, t.K, src, dst, n_ray, minv, b, counts, offsets, runs, n_runs);
+    else
+        hipLaunchKernelGGL((k_table_runs<false, FILL>), grid, block, 0, stream, dims, bg, t.tvals, t.u16, t.start,
+                           t.scale, t.K, src, dst, n_ray, minv, b, counts, offsets, runs, n_runs);
+    return check_launch(fn);
+}
+
+}  // namespace ctlab
+
+using namespace ctlab;
+
+extern "C" int64_t ctlab_table_brick_count(int64_t n_x, int64_t n_y, int64_t n_z) {
+    if (n_x < 1 || n_y < 1 || n_z < 1) return 0;
+    const int64_t n[3] = {n_x, n_y, n_z};
+    const BrickGrid bg = brick_grid(n);
+    return (int64_t)bg.nb[0] * bg.nb[1] * bg.nb[2];
+}
+
+extern "C" int ctlab_table_runs_count_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* tvals,
+                                         const uint16_t* u16, const float* start, const float* scale, int64_t K,
+                                         const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                         const float* b, uint32_t* counts, ctlab_stream_t stream) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    return table_runs<false>("ctlab_table_runs_count_3d", n, TableArgs{tvals, u16, start, scale, K}, src, dst, n_ray,
+                             M_inv, b, counts, nullptr, nullptr, 0, as_stream(stream));
+}
+
+extern "C" int ctlab_table_runs_fill_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* tvals,
+                                        const uint16_t* u16, const float* start, const float* scale, int64_t K,
+                                        const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                        const float* b, const int64_t* offsets, uint32_t* cursor,
+                                        ctlab_table_run_t* runs, int64_t n_runs, ctlab_stream_t stream) {
+    const int64_t n[3] = {n_x, n_y, n_z};
+    return table_runs<true>("ctlab_table_runs_fill_3d", n, TableArgs{tvals, u16, start, scale, K}, src, dst, n_ray,
+                            M_inv, b, cursor, offsets, runs, n_runs, as_stream(stream));
+}
+
+extern "C" int ctlab_back_project_3d_table_brick(const float* sino, int64_t batch, const float* tvals,
+                                                 const uint16_t* u16, const float* start, const float* scale,
+                                                 int64_t K, const float* src, const float* dst, int64_t n_ray,
+                                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y,
+                                                 int64_t n_z, const ctlab_table_run_t* runs, int64_t n_runs,
+                                                 const ctlab_table_unit_t* units, int64_t n_units, int accumulate,
+                                                 float* out, ctlab_stream_t stream) {
+    static const char* fn = "ctlab_back_project_3d_table_brick";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    const TableArgs t{tvals, u16, start, scale, K};
+    int st = check_table_geometry(fn, n, t, src, dst, n_ray, M_inv, b);
+    if (st) return st;
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range [1, 65535]", fn, (long long)batch);
+    CTLAB_REQUIRE(n_runs >= 0 && n_units >= 0 && n_units <= kMaxGridX, "%s: run/unit counts out of range", fn);
+    CTLAB_REQUIRE(out, "%s: out is null", fn);
+    const int64_t V = n_x * n_y * n_z;
+    hipStream_t s = as_stream(stream);
+    if (!accumulate) {
+        st = fill_f32(out, V * batch, 0.0f, s);
+        if (st) return st;
+    }
+    if (n_ray == 0 || n_units == 0 || K < 2) return CTLAB_OK;
+    CTLAB_REQUIRE(sino && runs && units, "%s: null pointer", fn);
+    Dims3 dims;
+    for (int a = 0; a < 3; ++a) dims.n[a] = (int)n[a];
+    const BrickGrid bg = brick_grid(n);
+    const dim3 grid((unsigned)n_units, (unsigned)batch), block(kTbBlock);
+    if (u16)
+        hipLaunchKernelGGL((k_back_table_brick<true>), grid, block, 0, s, sino, dims, bg, tvals, u16, start, scale, K,
+                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out);
+    else
+        hipLaunchKernelGGL((k_back_table_brick<false>), grid, block, 0, s, sino, dims, bg, tvals, u16, start, scale, K,
+                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out);
+    return check_launch(fn);
+}

@@ -186,8 +186,7 @@ __host__ __device__ __forceinline__ uint32_t mul_stride(uint32_t r, int stride) 
 // non-NaN operand) and is out of range, as round(NaN).long() is in the
 // reference.
 template <int ND>
-__host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const float* c, uint32_t* idx) {
-    int32_t r[ND];
+__host__ __device__ __forceinline__ bool round_voxel(const Grid<ND>& g, const float* c, int32_t* r) {
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
@@ -198,6 +197,13 @@ __host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const flo
         r[a] = (int32_t)fc;
         ok = ok && ((uint32_t)r[a] < (uint32_t)g.n[a]);
     }
+    return ok;
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const float* c, uint32_t* idx) {
+    int32_t r[ND];
+    const bool ok = round_voxel<ND>(g, c, r);
     if (ND == 3) {
         *idx = mad24((uint32_t)r[0], (uint32_t)(g.n[1] * g.n[2]), mad24((uint32_t)r[1], (uint32_t)g.n[2],
                                                                        (uint32_t)r[ND - 1]));
@@ -785,11 +791,13 @@ __host__ __device__ __forceinline__ void march_ray_2d(const Grid<2>& g, const fl
 // length |P1 - P0|, midpoint voxel by round-half-even.  p0 holds P(t0) on
 // entry and P(t1) on return.  Returns whether the voxel is in the grid; *idx
 // and *len receive the flat voxel index and the length.
+//
+// segment_mid is the same without the voxel lookup: it returns the length and
+// leaves the midpoint's continuous voxel coordinates in c.
 template <int ND, bool DIAG>
-__host__ __device__ __forceinline__ bool segment_from_t(const Grid<ND>& g, const MidIndex<ND, DIAG>& mi,
-                                                        const float* src, const float* vw, float* p0, float t1,
-                                                        uint32_t* idx, float* len) {
-    float p1[ND], c[ND];
+__host__ __device__ __forceinline__ float segment_mid(const MidIndex<ND, DIAG>& mi, const float* src,
+                                                      const float* vw, float* p0, float t1, float* c) {
+    float p1[ND];
     float ss = 0.0f;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
@@ -798,10 +806,19 @@ __host__ __device__ __forceinline__ bool segment_from_t(const Grid<ND>& g, const
         const float sq = dd * dd;
         ss = (a == 0) ? sq : ss + sq;
     }
-    *len = seg_sqrt(ss);
+    const float len = seg_sqrt(ss);
     mi.from_ends(p0, p1, c);
 #pragma unroll
     for (int a = 0; a < ND; ++a) p0[a] = p1[a];
+    return len;
+}
+
+template <int ND, bool DIAG>
+__host__ __device__ __forceinline__ bool segment_from_t(const Grid<ND>& g, const MidIndex<ND, DIAG>& mi,
+                                                        const float* src, const float* vw, float* p0, float t1,
+                                                        uint32_t* idx, float* len) {
+    float c[ND];
+    *len = segment_mid<ND, DIAG>(mi, src, vw, p0, t1, c);
     return flat_voxel<ND>(g, c, idx);
 }
 

@@ -152,19 +152,24 @@ def forward_project_3d_compressed_hip(volume, t_uint16, t_start, t_scale, M, b, 
 
 
 def back_project_3d_hip(sinogram, tvals, src, dst, M, b, n_x, n_y, n_z, *, semantics: str = "otf",
-                        det_shape=None, brick_plan=None):
+                        det_shape=None, brick_plan=None, table_runs=None):
     """Adjoint of ``forward_project_3d_hip`` with the same ``tvals``: [B, n_x, n_y, n_z]
     (or [n_x, n_y, n_z] for a 1D sinogram) (ct_projector_3d_cuda.py:186-225).
 
     With no table, ``brick_plan`` (from ``ConeBeamPlan``) selects the LDS-brick
-    back projector; otherwise the atomic one runs."""
+    back projector; otherwise the atomic one runs.  With a table,
+    ``table_runs`` (``_C.table_runs`` of that table, rays and map) selects the
+    table brick back projector; without it, large tables get cached run lists
+    and small ones the per-segment atomic kernels."""
     M, b = _collapse(M, b)
     if _empty_or_none(tvals):
         return back_project_3d_on_the_fly_hip(sinogram, src, dst, M, b, n_x, n_y, n_z, semantics=semantics,
                                               det_shape=det_shape, brick_plan=brick_plan)
     _check_inputs(sinogram=sinogram)
     M_inv = _prepare_M_inv(M)
-    if isinstance(tvals, (tuple, list)):
+    if table_runs is not None:
+        out = _C.back_project_3d_table_brick(sinogram, tvals, src, dst, table_runs, M_inv, b, n_x, n_y, n_z)
+    elif isinstance(tvals, (tuple, list)):
         t_u16, t_start, t_scale = tvals
         out = _C.back_project_3d_compressed_cuda(sinogram, t_u16, t_start, t_scale, src, dst, M_inv, b,
                                                  n_x, n_y, n_z)

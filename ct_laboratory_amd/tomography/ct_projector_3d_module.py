@@ -83,6 +83,7 @@ class CTProjector3DModule(torch.nn.Module):
         self.register_buffer("dst", dst.to(device).contiguous())
         self.tvals_uint16 = self.tvals_start = self.tvals_scale = self.tvals_full = None
         self._plan = None
+        self._runs = None
         self.precomputed_intersections = False
         if tvals is not None:
             self.set_tvals(tvals, device=device)
@@ -94,6 +95,7 @@ class CTProjector3DModule(torch.nn.Module):
             else:
                 self.tvals_full = tab
             self.precomputed_intersections = True
+            self._build_table_runs()
 
     # -- device moves -------------------------------------------------------
     def to(self, *args, **kwargs):
@@ -106,7 +108,23 @@ class CTProjector3DModule(torch.nn.Module):
                 if t is not None:
                     setattr(self, name, t.to(device))
             self._plan = None
+            self._build_table_runs()
         return self
+
+    def _build_table_runs(self):
+        """Run lists of the crossing table for the brick back projection, built
+        once per table (a geometry-only pass over it) when the table is large
+        enough for them to pay; None otherwise."""
+        self._runs = None
+        tv = self._table()
+        if tv is None or not self.src.is_cuda:
+            return
+        t0 = tv[0] if isinstance(tv, tuple) else tv
+        if t0.numel() < _C._MIN_TABLE_ENTRIES:
+            return
+        M, b = _collapse(self.M, self.b)
+        self._runs = _C.table_runs(tv, self.src, self.dst, _prepare_M_inv(M), b.to(torch.float32).contiguous(),
+                                   (self.n_x, self.n_y, self.n_z))
 
     # -- on-the-fly configuration -------------------------------------------
     def _options(self) -> OnTheFlyOptions:
@@ -138,9 +156,14 @@ class CTProjector3DModule(torch.nn.Module):
     def back_project(self, sinogram):
         tv = self._table()
         opts = self._options() if tv is None else OnTheFlyOptions(self.semantics)
+        runs = None
+        if tv is not None and self._runs is not None:
+            M, b = _collapse(self.M, self.b)
+            if self._runs.matches(tv, self.src, self.dst, _prepare_M_inv(M), b):
+                runs = self._runs
         return back_project_3d_hip(sinogram.contiguous(), tv, self.src, self.dst, self.M, self.b, self.n_x,
                                    self.n_y, self.n_z, semantics=opts.semantics, det_shape=opts.det_shape,
-                                   brick_plan=opts.plan)
+                                   brick_plan=opts.plan, table_runs=runs)
 
     # -- crossing tables ----------------------------------------------------
     def save_tvals(self, filePath):
@@ -166,6 +189,7 @@ class CTProjector3DModule(torch.nn.Module):
             t = d.get(name)
             setattr(self, name, None if t is None else t.to(device))
         self.precomputed_intersections = True
+        self._build_table_runs()
 
     def set_tvals(self, tvals, device=None):
         """Float table, (uint16, start, scale) tuple, or None for on-the-fly
@@ -174,6 +198,7 @@ class CTProjector3DModule(torch.nn.Module):
         if tvals is None:
             self.tvals_uint16 = self.tvals_start = self.tvals_scale = self.tvals_full = None
             self.precomputed_intersections = False
+            self._runs = None
             return
         if isinstance(tvals, (tuple, list)):
             tu16, tstart, tscale = tvals
@@ -191,6 +216,7 @@ class CTProjector3DModule(torch.nn.Module):
             self.tvals_full = tvals.to(device).to(torch.float32).contiguous()
             self.tvals_uint16 = self.tvals_start = self.tvals_scale = None
         self.precomputed_intersections = True
+        self._build_table_runs()
 
     @property
     def tvals(self):

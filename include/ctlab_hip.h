@@ -213,6 +213,69 @@ int ctlab_back_project_3d_u16(const float* sino, int64_t batch, const uint16_t* 
                               int accumulate, float* out, ctlab_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Table back projection in LDS bricks (same result as                    */
+/* ctlab_back_project_3d_tvals / ctlab_back_project_3d_u16 up to the       */
+/* grouping of float sums; the reference's consumers add every segment     */
+/* with a global atomicAdd, src/ct_projector_3d.cu:215-269, 325-376)       */
+/* ---------------------------------------------------------------------- */
+
+/* The volume is cut into bricks of X x Y x Z voxels, brick (i, j, k) holding
+ * voxels [32i, 32i+32) x [32j, 32j+32) x [16k, 16k+16), numbered
+ * (i * nb_y + j) * nb_z + k with nb_a = ceil(n_a / brick_a). */
+#define CTLAB_TABLE_BRICK_X 32
+#define CTLAB_TABLE_BRICK_Y 32
+#define CTLAB_TABLE_BRICK_Z 16
+
+/* A run: segments seg0 .. seg0+nseg-1 of ray `ray` (segment j runs from table
+ * entry j-1 to entry j), every visited one in the same brick; k0 is the uint16
+ * walk's running sum at entry seg0-1 (0 for float tables). */
+typedef struct ctlab_table_run {
+    uint32_t ray, seg0, nseg, k0;
+} ctlab_table_run_t;
+
+/* A work unit: runs first .. first+n-1, all of brick `brick`. */
+typedef struct ctlab_table_unit {
+    uint32_t brick, n;
+    int64_t first;
+} ctlab_table_unit_t;
+
+/* Number of bricks of an n_x x n_y x n_z grid. */
+int64_t ctlab_table_brick_count(int64_t n_x, int64_t n_y, int64_t n_z);
+
+/* Geometry-only run lists of a crossing table: pass exactly one of `tvals`
+ * (float [n_ray, K]) and `u16` (uint16 [n_ray, K] with start and scale [n_ray],
+ * CUDA decode).  The segments, voxels and skip rules are those of
+ * ctlab_back_project_3d_tvals / ctlab_back_project_3d_u16 (ZERO_THRESH grid).
+ *   1. ctlab_table_runs_count_3d: counts[brick] = number of runs (the call
+ *      clears counts first).
+ *   2. The caller forms offsets[brick] = exclusive prefix sum of counts and
+ *      n_runs = their total, and allocates runs [n_runs].
+ *   3. ctlab_table_runs_fill_3d places each run at offsets[brick] + a cursor
+ *      (the call clears cursor [n_brick] first); order within a brick is
+ *      unspecified.  A record whose slot would be >= n_runs is dropped. */
+int ctlab_table_runs_count_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* tvals, const uint16_t* u16,
+                              const float* start, const float* scale, int64_t K, const float* src, const float* dst,
+                              int64_t n_ray, const float* M_inv, const float* b, uint32_t* counts,
+                              ctlab_stream_t stream);
+int ctlab_table_runs_fill_3d(int64_t n_x, int64_t n_y, int64_t n_z, const float* tvals, const uint16_t* u16,
+                             const float* start, const float* scale, int64_t K, const float* src, const float* dst,
+                             int64_t n_ray, const float* M_inv, const float* b, const int64_t* offsets,
+                             uint32_t* cursor, ctlab_table_run_t* runs, int64_t n_runs, ctlab_stream_t stream);
+/* Back projection [batch, n_x, n_y, n_z] of sino [batch, n_ray] over the runs:
+ * one block per (unit, batch item) accumulates in an LDS brick and adds the
+ * brick's nonzero voxels to out with global atomics.  The table, rays, map and
+ * grid must be those the runs were built from; out is cleared first unless
+ * accumulate.  Same extension of the reference as ctlab_back_project_3d_u16 /
+ * ctlab_back_project_3d_tvals, whose calls it replaces when the run lists are
+ * at hand (bindings.cpp:87-88, 94-95). */
+int ctlab_back_project_3d_table_brick(const float* sino, int64_t batch, const float* tvals, const uint16_t* u16,
+                                      const float* start, const float* scale, int64_t K, const float* src,
+                                      const float* dst, int64_t n_ray, const float* M_inv, const float* b,
+                                      int64_t n_x, int64_t n_y, int64_t n_z, const ctlab_table_run_t* runs,
+                                      int64_t n_runs, const ctlab_table_unit_t* units, int64_t n_units,
+                                      int accumulate, float* out, ctlab_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Atomic-free back projection for flat-panel view stacks (no reference    */
 /* counterpart; an alternative schedule of back_project_3d_on_the_fly with  */
 /* identical segments)                                                      */
