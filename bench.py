@@ -200,8 +200,9 @@ def generic_bench(a, dev) -> int:
     """The 256^3 cone-beam workload's rays in a seeded random order, so that no
     detector layout describes them: the brick lists' build time, the back
     projection of a dense sinogram over brick lists and with one ray per lane
-    (global atomics), the forward in ray order, and the adjoint of the list
-    pair.  Prints one JSON line."""
+    (global atomics), the forward in storage order, the adjoint of the list
+    pair, and the _C path that sorts the rays into spatial order once.  Prints
+    one JSON line."""
     import torch
 
     from ct_laboratory_amd import _C
@@ -235,8 +236,18 @@ def generic_bench(a, dev) -> int:
     ws = torch.empty(1, device=dev)
     back_ms, bk = timed(lambda: _C.back_project_3d_brick_bins(y, ts, td, bins, tm, tb, n, n, n, mode=mode,
                                                                  workspace=ws), a.steps)
-    fwd_ms, sino = timed(lambda: _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb, mode=mode, det_shape=None),
-                         max(1, a.steps // 2))
+    fwd_ms, sino = timed(lambda: _C._forward_otf(tv, ts, td, tm, tb, mode, None, None), max(1, a.steps // 2))
+    # the _C path: spatial order built once (sort + reordered copies + lists), then
+    # the sinogram gathered into that order for each back projection
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    order = _C._order_of(ts, td)
+    _C._bins_of(order.src, order.dst, tm, tb, (n, n, n), mode)
+    torch.cuda.synchronize()
+    t_order = time.perf_counter() - t0
+    back_o_ms, bk_o = timed(lambda: _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, n, n, n, mode=mode), a.steps)
+    fwd_o_ms, sino_o = timed(lambda: _C.forward_project_3d_on_the_fly_cuda(tv, ts, td, tm, tb, mode=mode),
+                             max(1, a.steps // 2))
     _C.AUTO_LAYOUT = False
     try:
         atomic_ms, _ = timed(lambda: _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, n, n, n, mode=mode), 1)
@@ -250,6 +261,10 @@ def generic_bench(a, dev) -> int:
         "n_ray": n_ray, "pairs_listed": int(bins.ids.numel()), "pairs_per_ray": bins.ids.numel() / n_ray,
         "bins_build_s": t_bins, "back_bins_ms": back_ms, "back_atomic_ms": atomic_ms, "forward_ray_order_ms": fwd_ms,
         "adjoint_rel_err": abs(lhs - rhs) / abs(lhs),
+        "ordered": {"build_s": t_order, "back_ms": back_o_ms, "forward_ms": fwd_o_ms,
+                    "back_max_abs_diff_vs_unordered_lists": float((bk_o - bk).abs().max()),
+                    "back_max_abs": float(bk.abs().max()),
+                    "forward_equal_to_storage_order": bool(torch.equal(sino_o, sino))},
     }
     print(json.dumps(res), flush=True)
     return 0
@@ -320,6 +335,8 @@ def staticct_bench(a, dev) -> int:
     one JSON line."""
     import torch
 
+    from ct_laboratory_amd import _C
+    from ct_laboratory_amd import _native as N
     from ct_laboratory_amd.tomography.ct_projector_3d_module import CTProjector3DModule, precompute_tvals_stitched
 
     src, dst = staticct_ring_rays(dev)
@@ -337,10 +354,14 @@ def staticct_bench(a, dev) -> int:
             vals.append(1e3 * (time.perf_counter() - t0))
         return float(np.median(vals[1:])), out
 
-    rows = []
+    rows, segments = [], {}
     for res in a.ring_res:
         M, b, nz = ring_grid(res, dev)
         vol = torch.randn(res, res, nz, device=dev, generator=torch.Generator(device=dev).manual_seed(res))
+        # positive-length in-grid segments of the torch rule (the on-the-fly segments)
+        _, _, cnt = _C.trace_segments((res, res, nz), src, dst, _C.cached_inverse(M), b, mode=N.MODE_TORCH, cap=0)
+        segments[res] = int(cnt.to(torch.int64).sum().item())
+        del cnt
         ref = {}
         for label in ("uint16", "float", "none", "none+layout"):
             print(f"[staticct] res{res} {label}", file=sys.stderr, flush=True)
@@ -387,7 +408,7 @@ def staticct_bench(a, dev) -> int:
     res = {"metric": "StaticCT ring (240 views x 12,288 rays) CTProjector3DModule forward / back ms, median",
            "workload": "synthetic 80-source x 3-rotation ring, 16 of 48 modules of 48x16 px per view, "
                        "2,949,120 rays; 400 mm FOV grids with +5 mm z shift; randn volume",
-           "n_ray": n_ray, "repeats": repeats, "rows": rows}
+           "n_ray": n_ray, "segments": segments, "repeats": repeats, "rows": rows}
     print(json.dumps(res), flush=True)
     return 0
 

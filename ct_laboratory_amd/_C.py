@@ -118,7 +118,10 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
     """[B, n_ray] line integrals (src/ct_projector_3d.cu:1018-1049).
 
     ``det_shape=(rows, cols)`` declares rays ordered [view][rows][cols] (a
-    scheduling hint; results do not depend on it)."""
+    scheduling hint; results do not depend on it).  Without it, a recognised
+    layout schedules the same way, and a large ray set in no detector order is
+    traced in its cached spatial order (ray_order) with the results put back
+    in storage order."""
     name = "forward_project_3d_on_the_fly_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
@@ -126,7 +129,19 @@ def forward_project_3d_on_the_fly_cuda(volume, src, dst, M_inv, b, *, mode: int 
     _affine(name, M_inv, b, 3)
     if det_shape is None:
         det_shape = _layout_of(src, dst)
-    out = _out(name, out, (batch, n_ray), volume.device)
+        order = _order_of(src, dst) if det_shape is None else None
+        if order is not None:
+            res = _forward_otf(volume, order.src, order.dst, M_inv, b, mode, None, None)
+            out = _out(name, out, (batch, n_ray), volume.device)
+            return out.index_copy_(1, order.perm, res)
+    return _forward_otf(volume, src, dst, M_inv, b, mode, det_shape, out)
+
+
+def _forward_otf(volume, src, dst, M_inv, b, mode, det_shape, out):
+    """The forward launch over the rays in storage order (checked arguments)."""
+    batch, (nx, ny, nz) = _vol_dims("forward_project_3d_on_the_fly_cuda", volume, 3)
+    n_ray = int(src.shape[0])
+    out = _out("forward_project_3d_on_the_fly_cuda", out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         rows, cols = det_shape if det_shape is not None else (0, 0)
         N.call("ctlab_forward_project_3d_otf", N.ptr(volume), batch, nx, ny, nz, N.ptr(src), N.ptr(dst), n_ray,
@@ -141,8 +156,11 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
     Without ``det_shape``, rays recognised as a stack of flat-panel views are
     back-projected on LDS bricks over their detector rectangles, and other ray
     sets on LDS bricks over per-brick ray lists built once per ray set (same
-    segments either way); ``det_shape=(rows, cols)`` declares the layout and
-    keeps the one-ray-per-lane kernel, scheduled by it.  Under graph capture the
+    segments either way; a large unordered set is binned in its cached
+    spatial order, ray_order, with the sinogram gathered to match);
+    ``det_shape=(rows,
+    cols)`` declares the layout and keeps the one-ray-per-lane kernel,
+    scheduled by it.  Under graph capture the
     caches are not consulted (a lookup may synchronise) and the one-ray-per-lane
     kernel runs; a caller that captures passes its plan or lists to
     back_project_3d_brick / back_project_3d_brick_bins."""
@@ -156,6 +174,13 @@ def back_project_3d_on_the_fly_cuda(sino, src, dst, M_inv, b, n_x, n_y, n_z, *, 
         plan = _plan_of(src, dst, layout, M_inv, b, (int(n_x), int(n_y), int(n_z))) if layout else None
         if plan is not None:
             return back_project_3d_brick(sino, src, dst, layout, plan, M_inv, b, n_x, n_y, n_z, mode=mode, out=out)
+        order = _order_of(src, dst) if layout is None else None
+        if order is not None:
+            bins = _bins_of(order.src, order.dst, M_inv, b, (int(n_x), int(n_y), int(n_z)), mode)
+            if bins is not None:
+                s2 = sino.reshape(1, -1) if sino.dim() == 1 else sino
+                return back_project_3d_brick_bins(s2.index_select(1, order.perm), order.src, order.dst, bins, M_inv, b,
+                                                  n_x, n_y, n_z, mode=mode, out=out)
         bins = _bins_of(src, dst, M_inv, b, (int(n_x), int(n_y), int(n_z)), mode)
         if bins is not None:
             return back_project_3d_brick_bins(sino, src, dst, bins, M_inv, b, n_x, n_y, n_z, mode=mode, out=out)
@@ -989,6 +1014,89 @@ def _truns_of(table, src, dst, M_inv, b, shape):
             _TRUNS.move_to_end(key)
             return e[3]
     return table_runs(table, src, dst, M_inv, b, shape)
+
+
+# ---------------------------------------------------------------------------
+# Spatial order of ray sets in no detector order
+# ---------------------------------------------------------------------------
+
+ORDER_RAYS = os.environ.get("CTLAB_ORDER_RAYS", "1") != "0"
+_MIN_ORDER_RAYS = 1 << 20  # below this a ray set fits the caches in any order
+_ORDERS: OrderedDict = OrderedDict()  # ray key -> (src ref, dst ref, RayOrder)
+_ORDER_CHUNK = 1 << 24
+
+
+class RayOrder:
+    """A permutation of a ray set into spatial order: perm int64 [n_ray] (position
+    k of the ordered set is ray perm[k]) and the ordered src, dst copies."""
+
+    def __init__(self, perm, src, dst):
+        self.perm, self.src, self.dst = perm, src, dst
+
+
+def _spread10(x):
+    """Bits 0..9 of x moved to bits 0, 3, 6, ..., 27 (Morton interleave)."""
+    x = (x | (x << 16)) & 0x30000FF
+    x = (x | (x << 8)) & 0x300F00F
+    x = (x | (x << 4)) & 0x30C30C3
+    return (x | (x << 2)) & 0x9249249
+
+
+def _morton_keys(p, lo, span):
+    q = ((p - lo) / span * 1023.0).clamp_(0, 1023).to(torch.int64)
+    return _spread10(q[:, 0]) | (_spread10(q[:, 1]) << 1) | (_spread10(q[:, 2]) << 2)
+
+
+def ray_order(src, dst) -> RayOrder:
+    """Rays sorted by (Morton code of the source, Morton code of the detector
+    end), each on a 1024^3 lattice over the bounding box of all sources
+    (detector ends): rays from one source come together, neighbouring
+    detector ends next to each other.  A ray's forward result does not depend
+    on the order.  A back projection over brick lists sums each brick's rays in
+    fixed point; where a long list is split over several blocks, the parts'
+    float sums are added in an unspecified order, so ordered and unordered
+    lists agree to float rounding of those parts."""
+    name = "ray_order"
+    _check_f32(name, src, dst)
+    n = _rays(name, src, dst, 3)
+    keys = torch.empty(n, device=src.device, dtype=torch.int64)
+    bounds = []
+    for p in (src, dst):
+        fin = torch.isfinite(p).all(dim=1)
+        q = p[fin] if not bool(fin.all()) else p
+        lo = q.amin(dim=0) if q.numel() else torch.zeros(3, device=p.device)
+        hi = q.amax(dim=0) if q.numel() else torch.ones(3, device=p.device)
+        bounds.append((lo, (hi - lo).clamp_min(1e-30)))
+    for r0 in range(0, n, _ORDER_CHUNK):
+        r1 = min(n, r0 + _ORDER_CHUNK)
+        ks = _morton_keys(torch.nan_to_num(src[r0:r1]), *bounds[0])
+        kd = _morton_keys(torch.nan_to_num(dst[r0:r1]), *bounds[1])
+        keys[r0:r1] = (ks << 30) | kd
+    perm = torch.sort(keys, stable=True).indices
+    del keys
+    return RayOrder(perm, src.index_select(0, perm).contiguous(), dst.index_select(0, perm).contiguous())
+
+
+def _order_of(src, dst):
+    """Cached spatial order of a large ray set that no detector layout
+    describes; None for small sets, under graph capture, or when switched off."""
+    if not (AUTO_LAYOUT and ORDER_RAYS) or _capturing() or int(src.shape[0]) < _MIN_ORDER_RAYS:
+        return None
+    key = _ray_key(src, dst)
+    with _CACHE_LOCK:
+        e = _ORDERS.get(key)
+        if e is not None and _alive(e[0], src) and _alive(e[1], dst):
+            _ORDERS.move_to_end(key)
+            return e[2]
+    order = ray_order(src, dst)
+    with _CACHE_LOCK:
+        for k in [k for k, v in _ORDERS.items() if v[0]() is None or v[1]() is None]:
+            del _ORDERS[k]
+        _ORDERS[key] = (weakref.ref(src), weakref.ref(dst), order)
+        _ORDERS.move_to_end(key)
+        while len(_ORDERS) > _BINS_CACHE_SIZE:
+            _ORDERS.popitem(last=False)
+    return order
 
 
 def _plan_of(src, dst, layout, M_inv, b, shape):

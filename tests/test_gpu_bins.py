@@ -142,3 +142,51 @@ def test_c_path_lists_unstructured_rays(dev, mname, mode):
     scale = np.abs(ref).max()
     for arr in (got.cpu().numpy(), atomic.cpu().numpy()):
         assert np.all(np.abs(arr - ref) <= 1e-5 * (np.abs(ref) + 1e-3 * scale))
+
+
+@pytest.mark.parametrize("mname,mode", MODES)
+def test_c_path_orders_large_unstructured_sets(dev, mname, mode):
+    """A cone-beam scan of 1,228,800 rays in shuffled order: the _C path sorts it
+    once into spatial order (ray_order); the forward equals the storage-order
+    forward ray for ray (bit for bit), the back projection equals the brick
+    lists over the shuffled order to float rounding (each brick sums in fixed
+    point; a brick split over several blocks adds its parts' float sums), and
+    the oracle on a sample of rays."""
+    from ct_laboratory_amd.tomography.conebeam_projector_3d import build_conebeam_3d_geometry
+
+    shape = (96, 96, 64)
+    src, dst = build_conebeam_3d_geometry(120, 128, 80, 600.0, 900.0, det_spacing=1.2)
+    n = src.shape[0]
+    assert n >= _C._MIN_ORDER_RAYS
+    perm = torch.from_numpy(np.random.default_rng(260).permutation(n))
+    ts, td = src[perm].contiguous().to(dev), dst[perm].contiguous().to(dev)
+    b = np.array([-(k - 1) / 2 for k in shape], np.float32)
+    tm, tb = _t(np.eye(3), dev), _t(b, dev)
+    print(f"[bins] ordered path, {mname}: {n} shuffled rays on {shape}", flush=True)
+    order = _C.ray_order(ts, td)
+    assert torch.equal(torch.sort(order.perm).values, torch.arange(n, device=dev))
+    assert torch.equal(order.src, ts[order.perm]) and torch.equal(order.dst, td[order.perm])
+    vol = torch.rand(shape, device=dev, generator=torch.Generator(device=dev).manual_seed(261))
+    fwd = _C.forward_project_3d_on_the_fly_cuda(vol, ts, td, tm, tb, mode=mode)
+    assert _C._layout_of(ts, td) is None and _C._order_of(ts, td) is not None
+    plain = _C._forward_otf(vol, ts, td, tm, tb, mode, None, None)
+    assert torch.equal(fwd, plain)
+    y = torch.randn((1, n), device=dev, generator=torch.Generator(device=dev).manual_seed(262))
+    got = _C.back_project_3d_on_the_fly_cuda(y, ts, td, tm, tb, *shape, mode=mode)
+    bins = _C.brick_bins(ts, td, tm, tb, shape, mode=mode)
+    unordered = _C.back_project_3d_brick_bins(y, ts, td, bins, tm, tb, *shape, mode=mode)
+    mag = _C.back_project_3d_brick_bins(y.abs(), ts, td, bins, tm, tb, *shape, mode=mode)
+    diff = (got.double() - unordered.double()).abs()
+    assert bool((diff <= 1e-6 * mag.double() + 1e-30).all()), float((diff / mag.double().clamp_min(1e-30)).max())
+    pick = torch.from_numpy(np.sort(np.random.default_rng(263).choice(n, 4000, replace=False))).to(dev)
+    ys = torch.zeros_like(y)
+    ys[0, pick] = y[0, pick]
+    got_s = _C.back_project_3d_on_the_fly_cuda(ys, ts, td, tm, tb, *shape, mode=mode).cpu().numpy()
+    s_np, d_np, y_np = ts[pick].cpu().numpy(), td[pick].cpu().numpy(), ys[0, pick].cpu().numpy()[None]
+    ref = O.back(y_np, shape, np.eye(3, dtype=np.float32), b, s_np, d_np, mode=mname)
+    bound = O.back(np.abs(y_np), shape, np.eye(3, dtype=np.float32), b, s_np, d_np, mode=mname)
+    err = np.abs(got_s.astype(np.float64) - ref)
+    assert np.all(err <= 1e-5 * bound + 1e-30), float((err / np.maximum(bound, 1e-30)).max())
+    ref_f = O.forward(vol.cpu().numpy(), np.eye(3, dtype=np.float32), b, s_np, d_np, mode=mname)
+    f_np = fwd[0, pick].cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(f_np - ref_f) <= 1e-5 * np.maximum(np.abs(ref_f), 1e-3 * np.abs(ref_f).max()))
