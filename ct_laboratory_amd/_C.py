@@ -477,7 +477,8 @@ def back_project_3d_compressed_cuda(sino, t_uint16, t_start, t_scale, src, dst, 
 # Table back projection in LDS bricks (run lists of a crossing table)
 # ---------------------------------------------------------------------------
 
-TABLE_UNIT_RUNS = 16384  # runs per work unit of the table brick back projection
+TABLE_UNIT_RUNS = 8192  # runs per work unit of the table brick back projection
+SORT_TABLE_RUNS = "ray"  # runs of a brick in ray order ("length": longest first; None: fill order)
 
 
 class TableRuns:
@@ -547,6 +548,21 @@ def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_
         cursor = torch.empty((nbk,), device=dev, dtype=torch.int32)
         N.call("ctlab_table_runs_fill_3d", nx, ny, nz, tv, t16, ts, tsc, K, N.ptr(src), N.ptr(dst), n_ray,
                N.ptr(M_inv), N.ptr(b), N.ptr(offsets), N.ptr(cursor), N.ptr(runs), total, N.stream_of(src))
+        if SORT_TABLE_RUNS and total > 1:
+            # within a brick, runs in ray order: neighbouring lanes then walk neighbouring rays (on the
+            # 2.95 M-ray ring at 512^2 x 64 with per-entry loads, 11.5 ms against 11.9 in fill order and
+            # 13.0 longest first)
+            brick_of = torch.repeat_interleave(torch.arange(nbk, device=dev), torch.from_numpy(c).to(dev),
+                                               output_size=total)
+            if SORT_TABLE_RUNS == "length":
+                key = (brick_of << 32) | (0xFFFFFFFF - (runs[:total, 2].to(torch.int64) & 0xFFFFFFFF))
+            else:
+                key = (brick_of << 32) | (runs[:total, 0].to(torch.int64) & 0xFFFFFFFF)
+            del brick_of
+            order = torch.sort(key).indices
+            del key
+            runs[:total] = runs[:total].index_select(0, order)
+            del order
     # work units: slices of at most unit_runs runs of one brick, largest first
     u = max(1, int(unit_runs))
     bricks = np.nonzero(c)[0]

@@ -115,6 +115,39 @@ __global__ __launch_bounds__(kBlock) void k_table_runs(Dims3 dims, BrickGrid bg,
     if (active && cur != 0xffffffffu) close();
 }
 
+// Block q of 8 uint16 entries: entries [8q - off, 8q - off + 8) of the flat
+// table of n_ent entries, whose first entry sits `off` entries past a 16-byte
+// boundary.  One 16-byte load when the block lies wholly inside the table,
+// else its entries one by one (zero outside; those are never consumed).
+__device__ __forceinline__ uint4 load_block(const uint16_t* __restrict__ u16, int64_t q, int off, int64_t n_ent) {
+    const int64_t e0 = q * 8 - off;
+    if (e0 >= 0 && e0 + 8 <= n_ent) return *reinterpret_cast<const uint4*>(u16 + e0);
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    for (int i = 0; i < 8; ++i) {
+        const int64_t e = e0 + i;
+        if (e >= 0 && e < n_ent) w[i >> 1] |= (uint32_t)u16[e] << ((i & 1) * 16);
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Block q of 4 float entries: entries [4q - off, 4q - off + 4) of the flat
+// table of n_ent entries, whose first entry sits `off` entries past a 16-byte
+// boundary; loaded as load_block loads uint16 blocks.
+__device__ __forceinline__ float4 load_block4(const float* __restrict__ tv, int64_t q, int off, int64_t n_ent) {
+    const int64_t e0 = q * 4 - off;
+    if (e0 >= 0 && e0 + 4 <= n_ent) return *reinterpret_cast<const float4*>(tv + e0);
+    float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4; ++i) {
+        const int64_t e = e0 + i;
+        if (e >= 0 && e < n_ent) w[i] = tv[e];
+    }
+    return make_float4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ float pick4(const float4& b, int64_t gi) {
+    return (gi & 2) ? ((gi & 1) ? b.w : b.z) : ((gi & 1) ? b.y : b.x);
+}
+
 // The runs of one unit added into the LDS brick acc (origin x0, y0, z0).
 template <bool U16, bool DIAG>
 __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, int x0, int y0, int z0,
@@ -123,7 +156,8 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
                                                  const float* __restrict__ scale, int64_t K,
                                                  const float* __restrict__ src, const float* __restrict__ dst,
                                                  int64_t n_ray, const ctlab_table_run_t* __restrict__ runs,
-                                                 int64_t n_runs, const ctlab_table_unit_t& u, int64_t bi) {
+                                                 int64_t n_runs, const ctlab_table_unit_t& u, int64_t bi,
+                                                 int u16_off, int f32_off) {
     MidIndex<3, DIAG> mi;
     mi.init(g);
     const float* __restrict__ sn = sino + bi * n_ray;
@@ -155,10 +189,16 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
             const float t0 = (j0 == 1) ? st0 : __builtin_fmaf((float)rr.k0, sc, st0);
 #pragma unroll
             for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
-            const uint16_t* __restrict__ row = u16 + ray * K;
+            // entries read 8 at a time in 16-byte blocks (gi counts entries from the
+            // 16-byte boundary at or before the table's start)
+            const int64_t n_ent = n_ray * K;
+            int64_t gi = ray * K + j0 + u16_off;
+            uint4 blk = load_block(u16, gi >> 3, u16_off, n_ent);
             uint32_t ks = rr.k0;
-            for (int64_t j = j0; j < j1; ++j) {
-                ks += (uint32_t)row[j];
+            for (int64_t j = j0; j < j1; ++j, ++gi) {
+                if ((gi & 7) == 0 && j != j0) blk = load_block(u16, gi >> 3, u16_off, n_ent);
+                const uint32_t w = (gi & 4) ? ((gi & 2) ? blk.w : blk.z) : ((gi & 2) ? blk.y : blk.x);
+                ks += (gi & 1) ? (w >> 16) : (w & 0xffffu);
                 const float t1 = __builtin_fmaf((float)ks, sc, st0);
                 float c[3];
                 const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
@@ -166,12 +206,17 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
                 add(c, val * len);
             }
         } else {
-            const float* __restrict__ row = tvals + ray * K;
-            float t0 = row[j0 - 1];
+            // entries read 4 at a time in 16-byte blocks, as the uint16 ones
+            const int64_t n_ent = n_ray * K;
+            int64_t gi = ray * K + j0 - 1 + f32_off;
+            float4 blk = load_block4(tvals, gi >> 2, f32_off, n_ent);
+            float t0 = pick4(blk, gi);
 #pragma unroll
             for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
             for (int64_t j = j0; j < j1; ++j) {
-                const float t1 = row[j];
+                ++gi;
+                if ((gi & 3) == 0) blk = load_block4(tvals, gi >> 2, f32_off, n_ent);
+                const float t1 = pick4(blk, gi);
                 if (isinf(t0) || isinf(t1)) {  // end of a sorted row; a finite value restarts the walk
                     t0 = t1;
 #pragma unroll
@@ -193,7 +238,8 @@ __global__ __launch_bounds__(kTbBlock) void k_back_table_brick(
     const uint16_t* __restrict__ u16, const float* __restrict__ start, const float* __restrict__ scale, int64_t K,
     const float* __restrict__ src, const float* __restrict__ dst, int64_t n_ray, const float* __restrict__ minv,
     const float* __restrict__ b, const ctlab_table_run_t* __restrict__ runs, int64_t n_runs,
-    const ctlab_table_unit_t* __restrict__ units, int64_t vol_elems, float* __restrict__ out) {
+    const ctlab_table_unit_t* __restrict__ units, int64_t vol_elems, float* __restrict__ out, int u16_off,
+    int f32_off) {
     __shared__ float acc[kTbVox];
     const ctlab_table_unit_t u = units[blockIdx.x];
     const int64_t bi = blockIdx.y;
@@ -208,10 +254,10 @@ __global__ __launch_bounds__(kTbBlock) void k_back_table_brick(
     __syncthreads();
     if (g.diag)
         table_brick_runs<U16, true>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
-                                    n_runs, u, bi);
+                                    n_runs, u, bi, u16_off, f32_off);
     else
         table_brick_runs<U16, false>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
-                                     n_runs, u, bi);
+                                     n_runs, u, bi, u16_off, f32_off);
     __syncthreads();
     float* __restrict__ o = out + bi * vol_elems;
     for (int i = threadIdx.x; i < kTbVox; i += kTbBlock) {
@@ -340,15 +386,19 @@ extern "C" int ctlab_back_project_3d_table_brick(const float* sino, int64_t batc
     }
     if (n_ray == 0 || n_units == 0 || K < 2) return CTLAB_OK;
     CTLAB_REQUIRE(sino && runs && units, "%s: null pointer", fn);
+    CTLAB_REQUIRE(!u16 || ((uintptr_t)u16 & 1) == 0, "%s: the uint16 table is not 2-byte aligned", fn);
+    CTLAB_REQUIRE(!tvals || ((uintptr_t)tvals & 3) == 0, "%s: the float table is not 4-byte aligned", fn);
+    const int u16_off = u16 ? (int)(((uintptr_t)u16 & 15) >> 1) : 0;
+    const int f32_off = tvals ? (int)(((uintptr_t)tvals & 15) >> 2) : 0;
     Dims3 dims;
     for (int a = 0; a < 3; ++a) dims.n[a] = (int)n[a];
     const BrickGrid bg = brick_grid(n);
     const dim3 grid((unsigned)n_units, (unsigned)batch), block(kTbBlock);
     if (u16)
         hipLaunchKernelGGL((k_back_table_brick<true>), grid, block, 0, s, sino, dims, bg, tvals, u16, start, scale, K,
-                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out);
+                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out, u16_off, 0);
     else
         hipLaunchKernelGGL((k_back_table_brick<false>), grid, block, 0, s, sino, dims, bg, tvals, u16, start, scale, K,
-                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out);
+                           src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out, 0, f32_off);
     return check_launch(fn);
 }

@@ -86,6 +86,18 @@ def test_table_brick_vs_oracle_and_atomic(dev, kind, rotated):
         got = _C.back_project_3d_table_brick(_t(ys, dev), table, ts, td, runs, tm, tb, *shape)
         ok, worst = _within_bound(got.cpu().numpy(), ref, bound)
         assert ok, (unit_runs, worst)
+    # a table that does not start on a 16-byte boundary (a view 3 entries into a buffer)
+    t_dev = table[0] if kind == "u16" else table
+    buf = torch.empty(t_dev.numel() + 3, dtype=t_dev.dtype, device=dev)
+    buf[3:] = t_dev.reshape(-1)
+    view = buf[3:].view(t_dev.shape)
+    assert view.data_ptr() % 16 != 0
+    shifted = (view, table[1], table[2]) if kind == "u16" else view
+    for _ in range(1):
+        runs = _C.table_runs(shifted, ts, td, tm, tb, shape)
+        got = _C.back_project_3d_table_brick(_t(ys, dev), shifted, ts, td, runs, tm, tb, *shape)
+        ok, worst = _within_bound(got.cpu().numpy(), ref, bound)
+        assert ok, ("shifted", worst)
     # the per-segment atomic kernels on the same table
     if kind == "u16":
         atomic = _C.back_project_3d_compressed_cuda(_t(ys, dev), *table, ts, td, tm, tb, *shape)
