@@ -108,8 +108,13 @@ __device__ __forceinline__ void walk_u16(const Grid<3>& g, const float* s, const
         if (!active) continue;
         const int jn = (int)min((int64_t)kStageU16, K - c0);
         for (int j = (c0 == 0) ? 1 : 0; j < jn; ++j) {  // entry 0 holds no delta
+            const uint32_t dk = (uint32_t)mine[j];
+            // A zero delta repeats the previous t bit for bit: its segment has length
+            // exactly 0 (skipped below) and leaves P(t0) as it is.  The padding of a
+            // row (INF -> delta 0) is all such entries.
+            if (dk == 0u) continue;
             const uint32_t kp = ks;
-            ks += (uint32_t)mine[j];
+            ks += dk;
             // t1 = start + k * scale with the integer running sum k (3d.cu:128-129)
             const float t1 = __builtin_fmaf((float)ks, sc, st0);
             float c[3];
