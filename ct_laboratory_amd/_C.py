@@ -367,13 +367,18 @@ def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col, *, out=N
 
 
 def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b, *, out=None):
-    """[B, n_ray] line integrals from a crossing table (src/ct_projector_3d.cu:403-433)."""
+    """[B, n_ray] line integrals from a crossing table (src/ct_projector_3d.cu:403-433).
+    Tables of at least 4 M entries walk only each ray's visited range, from the
+    table's cached run lists (same result, bit for bit)."""
     name = "forward_project_3d_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
     n_ray = _rays(name, src, dst, 3)
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
+    runs = _truns_of(tvals, src, dst, M_inv, b, (nx, ny, nz))
+    if runs is not None:
+        return forward_project_3d_table_ranged(volume, tvals, src, dst, runs, M_inv, b, out=out)
     out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         N.call("ctlab_forward_project_3d_tvals", N.ptr(volume), batch, nx, ny, nz, N.ptr(tvals), K, N.ptr(src),
@@ -444,6 +449,10 @@ def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, 
     n_ray = _rays(name, src, dst, 3)
     K = _u16(name, t_uint16, t_start, t_scale, n_ray)
     _affine(name, M_inv, b, 3)
+    table = (t_uint16, t_start, t_scale)
+    runs = _truns_of(table, src, dst, M_inv, b, (nx, ny, nz))
+    if runs is not None:
+        return forward_project_3d_table_ranged(volume, table, src, dst, runs, M_inv, b, out=out)
     out = _out(name, out, (batch, n_ray), volume.device)
     with torch.cuda.device(volume.device):
         N.call("ctlab_forward_project_3d_u16", N.ptr(volume), batch, nx, ny, nz, N.ptr(t_uint16), N.ptr(t_start),
@@ -491,10 +500,35 @@ class TableRuns:
         self.runs, self.n_runs, self.units = runs, int(n_runs), units
         self.shape, self.K, self.n_ray, self.kind = tuple(int(x) for x in shape), int(K), int(n_ray), kind
         self._inputs = inputs
+        self._ranges = None
 
     @property
     def n_units(self) -> int:
         return int(self.units.shape[0])
+
+    def ray_ranges(self):
+        """(first, end, k_first) int32 [n_ray] per ray: its smallest first segment,
+        largest end and the running sum before the first (0, 0, 0 for a ray with
+        no visited segment); computed once from the runs."""
+        if self._ranges is None:
+            dev = self.runs.device
+            n = self.n_ray
+            first = torch.zeros(n, device=dev, dtype=torch.int32)
+            end = torch.zeros(n, device=dev, dtype=torch.int32)
+            kf = torch.zeros(n, device=dev, dtype=torch.int32)
+            if self.n_runs:
+                rr = self.runs[:self.n_runs].to(torch.int64) & 0xFFFFFFFF
+                ray = rr[:, 0]
+                key = torch.full((n,), 1 << 62, device=dev, dtype=torch.int64)
+                key.scatter_reduce_(0, ray, (rr[:, 1] << 32) | rr[:, 3], reduce="amin")
+                e = torch.zeros(n, device=dev, dtype=torch.int64)
+                e.scatter_reduce_(0, ray, rr[:, 1] + rr[:, 2], reduce="amax")
+                hit = key < (1 << 62)
+                first = torch.where(hit, key >> 32, 0).to(torch.int32)
+                kf = torch.where(hit, key & 0xFFFFFFFF, 0).to(torch.int32)  # uint32 bits
+                end = torch.where(hit, e, 0).to(torch.int32)
+            self._ranges = (first.contiguous(), end.contiguous(), kf.contiguous())
+        return self._ranges
 
     def matches(self, table, src, dst, M_inv, b) -> bool:
         """Whether these are the very table, ray and map tensors (unwritten
@@ -580,6 +614,31 @@ def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_
     if not _capturing():
         _remember_truns(table, src, dst, M_inv, b, shape, result)
     return result
+
+
+def forward_project_3d_table_ranged(volume, table, src, dst, runs, M_inv, b, *, out=None):
+    """[B, n_ray] line integrals from a crossing table over each ray's visited
+    range (from ``runs``, the table's run lists): bit for bit the result of
+    forward_project_3d_cuda / forward_project_3d_compressed_cuda, without
+    walking the entries before a ray's first or after its last visited
+    segment."""
+    name = "forward_project_3d_table_ranged"
+    _check_f32(name, volume, src, dst, M_inv, b)
+    batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
+    n_ray = _rays(name, src, dst, 3)
+    tv, t16, ts, tsc, K, kind = _table_ptrs(name, table, n_ray)
+    _affine(name, M_inv, b, 3)
+    _require(isinstance(runs, TableRuns) and runs.shape == (nx, ny, nz) and runs.n_ray == n_ray and runs.K == K
+             and runs.kind == kind and runs.matches(table, src, dst, M_inv, b),
+             f"{name}: the run lists were built for another table, grid, rays or map, or those tensors were "
+             "written since")
+    first, end, kf = runs.ray_ranges()
+    out = _out(name, out, (batch, n_ray), volume.device)
+    with torch.cuda.device(volume.device):
+        N.call("ctlab_forward_project_3d_table_ranged", N.ptr(volume), batch, nx, ny, nz, tv, t16, ts, tsc, K,
+               N.ptr(first), N.ptr(end), N.ptr(kf), N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b),
+               N.ptr(out), N.stream_of(volume))
+    return out
 
 
 def back_project_3d_table_brick(sino, table, src, dst, runs, M_inv, b, n_x, n_y, n_z, *, out=None,
@@ -986,7 +1045,7 @@ def _bins_of(src, dst, M_inv, b, shape, mode):
 
 
 _TRUNS: OrderedDict = OrderedDict()  # (table key, ray key, grid, M_inv and b values) -> (refs, TableRuns)
-_TRUNS_CACHE_SIZE = 2
+_TRUNS_CACHE_SIZE = 4
 _MIN_TABLE_ENTRIES = 1 << 22  # below this the per-segment atomic kernels are as fast
 
 

@@ -87,6 +87,8 @@ _SIGS = {
                                  _P],
     "ctlab_back_project_3d_table_brick": [_P, _I64, _P, _P, _P, _P, _I64, _P, _P, _I64, _P, _P, _I64, _I64, _I64,
                                           _P, _I64, _P, _I64, _INT, _P, _P],
+    "ctlab_forward_project_3d_table_ranged": [_P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _P, _P, _P, _P,
+                                              _P, _I64, _P, _P, _P, _P],
     "ctlab_version": [],
     "ctlab_last_error": [],
 }

@@ -24,6 +24,8 @@
 #include "siddon_traverse.h"
 #include "table_walk.h"
 
+#include <type_traits>
+
 namespace ctlab {
 
 constexpr int kTbX = CTLAB_TABLE_BRICK_X, kTbY = CTLAB_TABLE_BRICK_Y, kTbZ = CTLAB_TABLE_BRICK_Z;
@@ -269,6 +271,111 @@ __global__ __launch_bounds__(kTbBlock) void k_back_table_brick(
     }
 }
 
+// Forward projection from a table over each ray's visited range only: segments
+// first[r] .. end[r]-1, with k_first[r] the uint16 running sum at entry
+// first[r]-1 (0, 0 = no visited segment).  Outside that range no segment is
+// visited, so the sum -- the same fused multiply-adds over the same segments
+// in the same order as k_forward_u16 / k_forward_tvals -- is bit for bit
+// theirs.  The block stages only the row chunks its rays' ranges cover.
+template <bool U16>
+__global__ __launch_bounds__(kBlock) void k_forward_table_ranged(
+    const float* __restrict__ vol, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    const float* __restrict__ tvals, const uint16_t* __restrict__ u16, const float* __restrict__ start,
+    const float* __restrict__ scale, int64_t K, const int32_t* __restrict__ first, const int32_t* __restrict__ end,
+    const uint32_t* __restrict__ k_first, const float* __restrict__ src, const float* __restrict__ dst,
+    const float* __restrict__ minv, const float* __restrict__ b) {
+    using T = typename std::conditional<U16, uint16_t, float>::type;
+    constexpr int C = U16 ? kStageU16 : kStageF32;
+    __shared__ RowStage<T, C> st;
+    __shared__ int s_lo, s_hi;
+    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t r = r0 + threadIdx.x;
+    const bool active = r < n_ray;
+    const int64_t bi = blockIdx.y;
+    const int64_t jf = active ? first[r] : 0, je = active ? end[r] : 0;
+    const bool walk = active && jf >= 1 && jf < je && je <= K;
+    if (threadIdx.x == 0) {
+        s_lo = 0x7fffffff;
+        s_hi = 0;
+    }
+    __syncthreads();
+    if (walk) {
+        atomicMin(&s_lo, (int)jf);
+        atomicMax(&s_hi, (int)je);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;  // block-uniform
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const float* __restrict__ v = vol + bi * vol_elems;
+    float acc = 0.0f;
+    if (lo < hi && g.finite) {
+        float s[3] = {}, vw[3] = {}, p0[3] = {};
+        float st0 = 0.0f, sc = 0.0f, t0 = 0.0f;
+        uint32_t ks = 0;
+        if (walk) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                s[a] = src[r * 3 + a];
+                vw[a] = dst[r * 3 + a] - s[a];
+            }
+            if (U16) {
+                st0 = start[r];
+                sc = scale[r];
+                ks = k_first[r];
+                // t of entry first-1: start for entry 0, else start + k * scale (3d.cu:128-129)
+                t0 = (jf == 1) ? st0 : __builtin_fmaf((float)ks, sc, st0);
+            } else {
+                t0 = tvals[r * K + jf - 1];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
+        }
+        auto run = [&](auto diag) {
+            constexpr bool DIAG = decltype(diag)::value;
+            MidIndex<3, DIAG> mi;
+            mi.init(g);
+            const T* __restrict__ mine = st.e + threadIdx.x * RowStage<T, C>::kStride;
+            const T* __restrict__ table;
+            if constexpr (U16) table = u16; else table = tvals;
+            for (int64_t c0 = (int64_t)(lo / C) * C; c0 < hi; c0 += C) {
+                stage_rows(st, table, K, r0, n_ray, c0);
+                if (!walk) continue;
+                const int64_t j0 = max(c0, jf), j1 = min(min(c0 + C, je), K);
+                for (int64_t j = j0; j < j1; ++j) {
+                    float c[3];
+                    if constexpr (U16) {
+                        const uint32_t dk = (uint32_t)mine[j - c0];
+                        if (dk == 0u) continue;  // same t: a segment of length 0
+                        ks += dk;
+                        const float t1 = __builtin_fmaf((float)ks, sc, st0);
+                        const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
+                        if (!(len >= 1e-9f)) continue;  // 3d.cu:137
+                        uint32_t idx;
+                        if (flat_voxel<3>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+                    } else {
+                        const float t1 = mine[j - c0];
+                        if (isinf(t0) || isinf(t1)) {  // end of a sorted row; a finite value restarts the walk
+                            t0 = t1;
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
+                            continue;
+                        }
+                        const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
+                        t0 = t1;
+                        uint32_t idx;
+                        if (len > 0.0f && flat_voxel<3>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+                    }
+                }
+            }
+        };
+        if (g.diag)
+            run(std::integral_constant<bool, true>());
+        else
+            run(std::integral_constant<bool, false>());
+    }
+    if (active) out[bi * n_ray + r] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -400,5 +507,32 @@ extern "C" int ctlab_back_project_3d_table_brick(const float* sino, int64_t batc
     else
         hipLaunchKernelGGL((k_back_table_brick<false>), grid, block, 0, s, sino, dims, bg, tvals, u16, start, scale, K,
                            src, dst, n_ray, M_inv, b, runs, n_runs, units, V, out, 0, f32_off);
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_forward_project_3d_table_ranged(const float* volume, int64_t batch, int64_t n_x, int64_t n_y,
+                                                    int64_t n_z, const float* tvals, const uint16_t* u16,
+                                                    const float* start, const float* scale, int64_t K,
+                                                    const int32_t* first, const int32_t* end,
+                                                    const uint32_t* k_first, const float* src, const float* dst,
+                                                    int64_t n_ray, const float* M_inv, const float* b, float* out,
+                                                    ctlab_stream_t stream) {
+    static const char* fn = "ctlab_forward_project_3d_table_ranged";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_table_geometry(fn, n, TableArgs{tvals, u16, start, scale, K}, src, dst, n_ray, M_inv, b);
+    if (st) return st;
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range [1, 65535]", fn, (long long)batch);
+    if (n_ray == 0) return CTLAB_OK;
+    CTLAB_REQUIRE(volume && out && first && end && (!u16 || k_first), "%s: null pointer", fn);
+    Dims3 dims;
+    for (int a = 0; a < 3; ++a) dims.n[a] = (int)n[a];
+    const int64_t V = n_x * n_y * n_z;
+    const dim3 grid(grid_for(n_ray), (unsigned)batch);
+    if (u16)
+        hipLaunchKernelGGL((k_forward_table_ranged<true>), grid, dim3(kBlock), 0, as_stream(stream), volume, out,
+                           n_ray, V, dims, tvals, u16, start, scale, K, first, end, k_first, src, dst, M_inv, b);
+    else
+        hipLaunchKernelGGL((k_forward_table_ranged<false>), grid, dim3(kBlock), 0, as_stream(stream), volume, out,
+                           n_ray, V, dims, tvals, u16, start, scale, K, first, end, k_first, src, dst, M_inv, b);
     return check_launch(fn);
 }

@@ -275,6 +275,22 @@ int ctlab_back_project_3d_table_brick(const float* sino, int64_t batch, const fl
                                       int64_t n_runs, const ctlab_table_unit_t* units, int64_t n_units,
                                       int accumulate, float* out, ctlab_stream_t stream);
 
+/* Forward projection [batch, n_ray] from a crossing table (tvals, or u16 with
+ * start and scale, as for the run lists) over each ray's visited range only:
+ * segments first[r] .. end[r]-1 (first = end = 0: none), k_first[r] the uint16
+ * running sum at entry first[r]-1 (unused for float tables).  The ranges come
+ * from the table's run lists (the smallest first segment and the largest end
+ * of a ray's runs).  No segment outside its range is visited, so out equals
+ * ctlab_forward_project_3d_u16 / ctlab_forward_project_3d_tvals bit for bit
+ * (same segments, same order, same fused multiply-adds); whose calls it
+ * replaces when the ranges are at hand (bindings.cpp:85-86, 92-93). */
+int ctlab_forward_project_3d_table_ranged(const float* volume, int64_t batch, int64_t n_x, int64_t n_y, int64_t n_z,
+                                          const float* tvals, const uint16_t* u16, const float* start,
+                                          const float* scale, int64_t K, const int32_t* first, const int32_t* end,
+                                          const uint32_t* k_first, const float* src, const float* dst,
+                                          int64_t n_ray, const float* M_inv, const float* b, float* out,
+                                          ctlab_stream_t stream);
+
 /* ---------------------------------------------------------------------- */
 /* Atomic-free back projection for flat-panel view stacks (no reference    */
 /* counterpart; an alternative schedule of back_project_3d_on_the_fly with  */

@@ -159,6 +159,44 @@ def test_table_brick_accumulate_and_cache(dev):
         _C.back_project_3d_table_brick(y, table, ts, td, runs, tm, tb, shape[0] + 1, shape[1], shape[2])
 
 
+@pytest.mark.parametrize("rotated", [False, True])
+@pytest.mark.parametrize("kind", ["f32", "u16"])
+def test_ranged_forward_equals_full_walk(dev, kind, rotated):
+    """The forward over each ray's visited range (from the run lists) equals the
+    forward that walks whole rows bit for bit, and the oracle's table forward."""
+    rng = np.random.default_rng(740 + 2 * rotated + (kind == "u16"))
+    shape, M, minv, b, src, dst = _case(rng, rotated)
+    tab = O.intersections(shape, minv, b, src, dst)
+    ts, td, tm, tb = _t(src, dev), _t(dst, dev), _t(minv, dev), _t(b, dev)
+    vol = rng.standard_normal((2,) + shape).astype(np.float32)
+    tv = _t(vol, dev)
+    if kind == "u16":
+        u, st, sc = O.compress_cuda(tab, src, dst, O.voxel_diag_len(M))
+        table = (_t(u.view(np.int16), dev), _t(st, dev), _t(sc, dev))
+        ref = O.forward_u16(vol, u, st, sc, minv, b, src, dst)
+        bound = O.forward_u16(np.abs(vol), u, st, sc, minv, b, src, dst)
+    else:
+        table = _t(tab, dev)
+        ref = O.forward_tvals(vol, tab, minv, b, src, dst)
+        bound = O.forward_tvals(np.abs(vol), tab, minv, b, src, dst)
+    runs = _C.table_runs(table, ts, td, tm, tb, shape)
+    first, end, _ = runs.ray_ranges()
+    assert int((end > first).sum()) > 0 and int(end.max()) <= tab.shape[1]
+    got = _C.forward_project_3d_table_ranged(tv, table, ts, td, runs, tm, tb)
+    _C.AUTO_LAYOUT = False  # the whole-row kernels
+    try:
+        if kind == "u16":
+            full = _C.forward_project_3d_compressed_cuda(tv, *table, ts, td, tm, tb)
+        else:
+            full = _C.forward_project_3d_cuda(tv, table, ts, td, tm, tb)
+    finally:
+        _C.AUTO_LAYOUT = True
+    assert torch.equal(got, full)
+    # signed values cancel along a ray: the rounding bound scales with the integral of |vol|
+    ok, worst = _within_bound(got.cpu().numpy(), ref, bound)
+    assert ok, worst
+
+
 @pytest.fixture(scope="module")
 def ring512(dev):
     import bench
@@ -205,6 +243,20 @@ def test_ring512_brick_vs_atomic_dense(dev, ring512):
     # both sides are float32 sums: allow the rounding of each (2e-5 of the magnitude sum)
     err = (got.double() - atomic.double()).abs()
     assert bool((err <= 2e-5 * bound.double() + 1e-30).all()), float((err / bound.double().clamp_min(1e-30)).max())
+
+
+def test_ring512_ranged_forward_equals_full_walk(dev, ring512):
+    g = ring512
+    vol = torch.randn(g["shape"], device=dev, generator=torch.Generator(device=dev).manual_seed(4))
+    print("[table_brick] ring res512: ranged forward vs whole-row forward", flush=True)
+    runs = _C.table_runs(g["table"], g["src"], g["dst"], g["minv"], g["b"], g["shape"])
+    got = _C.forward_project_3d_table_ranged(vol, g["table"], g["src"], g["dst"], runs, g["minv"], g["b"])
+    _C.AUTO_LAYOUT = False
+    try:
+        full = _C.forward_project_3d_compressed_cuda(vol, *g["table"], g["src"], g["dst"], g["minv"], g["b"])
+    finally:
+        _C.AUTO_LAYOUT = True
+    assert torch.equal(got, full)
 
 
 def test_ring512_brick_vs_oracle_sample(dev, ring512):
