@@ -553,11 +553,13 @@ def _table_inputs(table, src, dst, M_inv, b):
             _map_values(M_inv), _map_values(b))
 
 
-def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_RUNS):
+def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_RUNS, max_bytes=None):
     """Run lists of ``table`` (float [n_ray, K], or the (uint16, start, scale)
     tuple of the CUDA decode) for these rays, map and grid: two passes over the
     table and one synchronisation (the brick counts come to the host to size
-    the run list and cut the work units)."""
+    the run list and cut the work units).  With ``max_bytes``, returns None when
+    the lists and the temporaries of their ordering (48 bytes per run) would
+    need more device memory than that."""
     name = "table_runs"
     _check_f32(name, src, dst, M_inv, b)
     n_ray = _rays(name, src, dst, 3)
@@ -577,6 +579,8 @@ def table_runs(table, src, dst, M_inv, b, shape, *, unit_runs: int = TABLE_UNIT_
         offsets_h = np.zeros(nbk + 1, dtype=np.int64)
         np.cumsum(c, out=offsets_h[1:])
         total = int(offsets_h[-1])
+        if max_bytes is not None and 48 * total > max_bytes:
+            return None
         offsets = torch.from_numpy(offsets_h).to(dev)
         runs = torch.empty((max(total, 1), 4), device=dev, dtype=torch.int32)
         cursor = torch.empty((nbk,), device=dev, dtype=torch.int32)
@@ -1088,7 +1092,9 @@ def _truns_of(table, src, dst, M_inv, b, shape):
                 and all(_alive(r, p) for r, p in zip(e[2], parts))):
             _TRUNS.move_to_end(key)
             return e[3]
-    return table_runs(table, src, dst, M_inv, b, shape)
+    # the lists must leave room: at most half of the free device memory
+    free, _ = torch.cuda.mem_get_info(src.device)
+    return table_runs(table, src, dst, M_inv, b, shape, max_bytes=free // 2)
 
 
 # ---------------------------------------------------------------------------

@@ -123,8 +123,10 @@ class CTProjector3DModule(torch.nn.Module):
         if t0.numel() < _C._MIN_TABLE_ENTRIES:
             return
         M, b = _collapse(self.M, self.b)
+        free, _ = torch.cuda.mem_get_info(self.src.device)
+        # None when the lists would take more than half of the free device memory
         self._runs = _C.table_runs(tv, self.src, self.dst, _prepare_M_inv(M), b.to(torch.float32).contiguous(),
-                                   (self.n_x, self.n_y, self.n_z))
+                                   (self.n_x, self.n_y, self.n_z), max_bytes=free // 2)
 
     # -- on-the-fly configuration -------------------------------------------
     def _options(self) -> OnTheFlyOptions:

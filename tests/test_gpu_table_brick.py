@@ -157,6 +157,9 @@ def test_table_brick_accumulate_and_cache(dev):
         _C._MIN_TABLE_ENTRIES = old
     with pytest.raises(RuntimeError):
         _C.back_project_3d_table_brick(y, table, ts, td, runs, tm, tb, shape[0] + 1, shape[1], shape[2])
+    # lists that would not fit the given memory are not built (the callers then use the atomic kernels)
+    assert _C.table_runs(table, ts, td, tm, tb, shape, max_bytes=48 * runs.n_runs - 1) is None
+    assert _C.table_runs(table, ts, td, tm, tb, shape, max_bytes=48 * runs.n_runs) is not None
 
 
 @pytest.mark.parametrize("rotated", [False, True])
