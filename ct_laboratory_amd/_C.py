@@ -368,15 +368,17 @@ def back_project_2d_cuda(sino, tvals, src, dst, M_inv, b, n_row, n_col, *, out=N
 
 def forward_project_3d_cuda(volume, tvals, src, dst, M_inv, b, *, out=None):
     """[B, n_ray] line integrals from a crossing table (src/ct_projector_3d.cu:403-433).
-    Tables of at least 4 M entries walk only each ray's visited range, from the
-    table's cached run lists (same result, bit for bit)."""
+    When run lists of this table are cached (a back projection or the module
+    built them), only each ray's visited range is walked (same result, bit for
+    bit)."""
     name = "forward_project_3d_cuda"
     _check_f32(name, volume, src, dst, M_inv, b)
     batch, (nx, ny, nz) = _vol_dims(name, volume, 3)
     n_ray = _rays(name, src, dst, 3)
     K = _tvals(name, tvals, n_ray)
     _affine(name, M_inv, b, 3)
-    runs = _truns_of(tvals, src, dst, M_inv, b, (nx, ny, nz))
+    # lists already built for this table (a forward alone would not repay building them)
+    runs = _truns_of(tvals, src, dst, M_inv, b, (nx, ny, nz), build=False)
     if runs is not None:
         return forward_project_3d_table_ranged(volume, tvals, src, dst, runs, M_inv, b, out=out)
     out = _out(name, out, (batch, n_ray), volume.device)
@@ -450,7 +452,7 @@ def forward_project_3d_compressed_cuda(volume, t_uint16, t_start, t_scale, src, 
     K = _u16(name, t_uint16, t_start, t_scale, n_ray)
     _affine(name, M_inv, b, 3)
     table = (t_uint16, t_start, t_scale)
-    runs = _truns_of(table, src, dst, M_inv, b, (nx, ny, nz))
+    runs = _truns_of(table, src, dst, M_inv, b, (nx, ny, nz), build=False)
     if runs is not None:
         return forward_project_3d_table_ranged(volume, table, src, dst, runs, M_inv, b, out=out)
     out = _out(name, out, (batch, n_ray), volume.device)
@@ -1075,11 +1077,11 @@ def _remember_truns(table, src, dst, M_inv, b, shape, runs):
             _TRUNS.popitem(last=False)
 
 
-def _truns_of(table, src, dst, M_inv, b, shape):
-    """Run lists of a crossing table (the last ones table_runs built for it, or
-    new ones), kept per table, ray set, map values and grid while those tensors
-    are alive and unwritten; None while a graph is being captured (building
-    them synchronises) or for small tables."""
+def _truns_of(table, src, dst, M_inv, b, shape, build: bool = True):
+    """Run lists of a crossing table (the last ones table_runs built for it, or,
+    with ``build``, new ones), kept per table, ray set, map values and grid
+    while those tensors are alive and unwritten; None while a graph is being
+    captured (building them synchronises) or for small tables."""
     if not AUTO_LAYOUT or _capturing():
         return None
     t0 = table[0] if isinstance(table, (tuple, list)) else table
@@ -1092,6 +1094,8 @@ def _truns_of(table, src, dst, M_inv, b, shape):
                 and all(_alive(r, p) for r, p in zip(e[2], parts))):
             _TRUNS.move_to_end(key)
             return e[3]
+    if not build:
+        return None
     # the lists must leave room: at most half of the free device memory
     free, _ = torch.cuda.mem_get_info(src.device)
     return table_runs(table, src, dst, M_inv, b, shape, max_bytes=free // 2)

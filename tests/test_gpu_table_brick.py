@@ -152,7 +152,10 @@ def test_table_brick_accumulate_and_cache(dev):
         r1 = _C._truns_of(table, ts, td, tm, tb, shape)
         assert r1 is _C._truns_of(table, ts, td, tm, tb, shape)
         table.add_(0.0)
-        assert _C._truns_of(table, ts, td, tm, tb, shape) is not r1
+        # a lookup alone (as the table forwards make it) builds nothing for the written table
+        assert _C._truns_of(table, ts, td, tm, tb, shape, build=False) is None
+        r2 = _C._truns_of(table, ts, td, tm, tb, shape)
+        assert r2 is not r1 and _C._truns_of(table, ts, td, tm, tb, shape, build=False) is r2
     finally:
         _C._MIN_TABLE_ENTRIES = old
     with pytest.raises(RuntimeError):
