@@ -1041,6 +1041,10 @@ def _bins_of(src, dst, M_inv, b, shape, mode):
         if e is not None and _alive(e[0], src) and _alive(e[1], dst):
             _BINS.move_to_end(key)
             return e[2]
+    # lists of about six (brick, ray) pairs per ray, 4 bytes each, plus the counts
+    free, _ = torch.cuda.mem_get_info(src.device)
+    if 32 * int(src.shape[0]) > free // 2:
+        return None
     bins = brick_bins(src, dst, M_inv, b, shape, mode=mode)
     with _CACHE_LOCK:
         _BINS[key] = (weakref.ref(src), weakref.ref(dst), bins)
@@ -1173,6 +1177,10 @@ def _order_of(src, dst):
         if e is not None and _alive(e[0], src) and _alive(e[1], dst):
             _ORDERS.move_to_end(key)
             return e[2]
+    # reordered copies, permutation and sort keys: about 48 bytes per ray
+    free, _ = torch.cuda.mem_get_info(src.device)
+    if 48 * int(src.shape[0]) > free // 2:
+        return None
     order = ray_order(src, dst)
     with _CACHE_LOCK:
         for k in [k for k, v in _ORDERS.items() if v[0]() is None or v[1]() is None]:
