@@ -541,6 +541,34 @@ def serve_bench(a, dev) -> int:
 
     worst = max(float((served[r.rid][0] - plain[r.rid][0]).abs().max() / (plain[r.rid][0].abs().max() + 1e-30))
                 for r in reqs)
+    ref = None
+    if a.ref_plain:
+        # the first requests as one burst (all arrived at 0): the server, the plain loop, and the
+        # plain loop on the reference's kernel structure (table kernels with per-segment atomics
+        # for fan-beam, storage-order on-the-fly kernels with atomics for cone-beam)
+        import dataclasses
+        sub = [dataclasses.replace(r, arrival=0.0) for r in sorted(reqs, key=lambda r: r.arrival)[:a.ref_plain]]
+        plain_loop(warm, dev, reference_kernels=True)
+        print(f"[serve] burst of {len(sub)}: server", file=sys.stderr, flush=True)
+        mark()
+        b_served, b_span_s = ReconServer(dev, hbm_budget=200e9, slots=a.slots).run(sub)
+        mark()
+        print(f"[serve] burst of {len(sub)}: plain loop", file=sys.stderr, flush=True)
+        b_plain, b_span_p = plain_loop(sub, dev)
+        mark()
+        print(f"[serve] burst of {len(sub)}: plain loop, reference kernel structure", file=sys.stderr, flush=True)
+        b_ref, b_span_r = plain_loop(sub, dev, reference_kernels=True)
+        mark()
+        ref = {
+            "n_requests": len(sub),
+            "mix": {"fan2d": sum(r.kind == "fan2d" for r in sub), "cone3d": sum(r.kind == "cone3d" for r in sub)},
+            "server_s": b_span_s, "plain_s": b_span_p, "plain_reference_kernels_s": b_span_r,
+            "throughput_ratio": b_span_p / b_span_s,
+            "throughput_ratio_vs_reference_kernels": b_span_r / b_span_s,
+            "max_rel_diff_reference_kernels_vs_plain": max(
+                float((b_ref[r.rid][0] - b_plain[r.rid][0]).abs().max() / (b_plain[r.rid][0].abs().max() + 1e-30))
+                for r in sub),
+        }
     out = {
         "metric": "served MAP stream: requests/s and p50/p99 latency, continuous batching vs plain per-request loop",
         "n_requests": len(reqs), "qps": a.qps, "slots": a.slots, "hbm_budget_gb": 200,
@@ -552,6 +580,8 @@ def serve_bench(a, dev) -> int:
         "throughput_ratio": span_p / span_s,
         "max_rel_diff_server_vs_plain": worst,
     }
+    if ref is not None:
+        out["burst"] = ref
     print(json.dumps(out), flush=True)
     return 0
 
@@ -645,6 +675,9 @@ def trace_summary(path: str) -> int:
         print(json.dumps({"error": f"expected 3 phase markers, found {len(marks)}"}))
         return 1
     phases = {"server": (marks[0][2], marks[1][1]), "plain": (marks[1][2], marks[2][1])}
+    if len(marks) >= 7:  # a --ref-plain burst: server, plain loop, plain loop on the reference's kernel structure
+        for i, name in enumerate(("burst_server", "burst_plain", "burst_plain_reference_kernels")):
+            phases[name] = (marks[3 + i][2], marks[4 + i][1])
     out = {}
     for name, (t0, t1) in phases.items():
         ks = [r for r in rows if r[1] >= t0 and r[2] <= t1]
@@ -722,6 +755,9 @@ def main(argv=None) -> int:
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--ref-plain", type=int, default=0, metavar="N",
+                    help="with --serve: also run the first N requests as one burst through the server, the plain "
+                         "loop and the plain loop on the reference's kernel structure")
     a = ap.parse_args(argv)
     if a.trace_summary:
         return trace_summary(a.trace_summary)

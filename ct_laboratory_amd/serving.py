@@ -18,7 +18,10 @@ admitted at once. Every iterate stays resident until the request completes.
 
 ``plain_loop`` is the comparison: one request at a time, the reference-style
 autograd loop with one host synchronisation per iteration, run over the same
-projector, preconditioner and iterations, so the final images agree.
+projector, preconditioner and iterations, so the final images agree.  With
+``reference_kernels=True`` it runs the same loop on the reference's kernel
+structure (table kernels and storage-order on-the-fly kernels with per-segment
+global atomics), a second baseline.
 """
 from __future__ import annotations
 
@@ -66,6 +69,31 @@ def _projector(req: MAPRequest, device):
     # 1 mm voxels, 1.5 mm pixels, SID 1000 / SDD 1500 (the cone-beam configuration)
     return ConeBeam3DProjector(req.n, req.n, req.n, req.n_view, req.det, req.det, 1000.0, 1500.0, det_spacing=1.5,
                                voxel_size=1.0, device=device)
+
+
+def _reference_projector(req: MAPRequest, device):
+    """The request's projector with the reference's kernel structure, for the
+    baseline loop: a fan-beam module holding its crossing table, consumed by the
+    table kernels (the back projection adds every segment with a global atomic,
+    2d.cu:117-155), and a cone-beam module without a detector layout, traced one
+    ray per lane in storage order with per-segment global atomics (the
+    structure of 3d.cu:821-1079).  The segments are those of ``_projector``'s
+    (torch rule).  Use with ``_C.AUTO_LAYOUT`` off, so that the shim does not
+    swap in its own schedules."""
+    from .tomography import CTProjector3DModule, FanBeam2DProjector
+    from .tomography.conebeam_projector_3d import build_conebeam_3d_geometry
+    from .tomography.standard_image_transform import standard_image_transform_3d
+
+    if req.kind == "fan2d":
+        proj = FanBeam2DProjector(req.n, req.n, req.n_view, req.det, 1000.0, 1500.0, det_spacing=1.0,
+                                  pixel_size=1.0, device=device, precomputed_intersections=True)
+        # a copy of the table (as loaded from disk) is consumed as given, by the table kernels
+        proj.tvals = proj.tvals.clone()
+        return proj
+    src, dst = build_conebeam_3d_geometry(req.n_view, req.det, req.det, 1000.0, 1500.0, det_spacing=1.5,
+                                          device=device)
+    M, b = standard_image_transform_3d(n_x=req.n, n_y=req.n, n_z=req.n, s_x=1.0, s_y=1.0, s_z=1.0)
+    return CTProjector3DModule(req.n, req.n, req.n, M.to(device), b.to(device), src, dst, device=device)
 
 
 def build_problem(req: MAPRequest, device) -> None:
@@ -165,18 +193,32 @@ class ReconServer:
         return results, time.perf_counter() - t0
 
 
-def plain_loop(requests, device=None):
+def plain_loop(requests, device=None, reference_kernels: bool = False):
     """One request at a time with the reference-style autograd MAP loop (one host
-    synchronisation per iteration), same projector, preconditioner and
-    iterations as the server."""
+    synchronisation per iteration), same preconditioner and iterations as the
+    server.  The projectors are the server's, or with ``reference_kernels``
+    those of ``_reference_projector`` (the reference's kernel structure, with
+    the shim's schedules switched off for the loop's duration)."""
+    from . import _C
+
     device = torch.device("cuda") if device is None else torch.device(device)
+    saved = _C.AUTO_LAYOUT
+    if reference_kernels:
+        _C.AUTO_LAYOUT = False
+    try:
+        return _plain_loop(requests, device, _reference_projector if reference_kernels else _projector)
+    finally:
+        _C.AUTO_LAYOUT = saved
+
+
+def _plain_loop(requests, device, make_projector):
     results = {}
     t0 = time.perf_counter()
     for req in sorted(requests, key=lambda r: r.arrival):
         now = time.perf_counter() - t0
         if req.arrival > now:
             time.sleep(req.arrival - now)
-        proj = _projector(req, device)
+        proj = make_projector(req, device)
         d2 = sqs_preconditioner(ProjectorOps(proj), req.var, req.beta)
         est = MaximumAPosterioriEstimator(LinearGaussianRandomVariable(proj, req.var),
                                           QuadraticSmoothnessRandomVariable(1.0), req.y,

@@ -56,3 +56,28 @@ def test_budget_limits_concurrency(dev):
     for r in reqs:
         a, b = out[r.rid][0].cpu().numpy(), ref[r.rid][0].cpu().numpy()
         assert rel_err(a, b) < 1e-5
+
+
+def test_plain_loop_on_reference_kernel_structure(dev):
+    """The baseline loop on the reference's kernel structure (fan-beam through a
+    table consumed as given, cone-beam without a detector layout, the shim's
+    schedules off) reconstructs the same images as the plain loop within 1e-4
+    (the segments are the same; atomics sum them in other orders), and leaves
+    the shim's schedule switch as it found it."""
+    from ct_laboratory_amd import _C
+    from ct_laboratory_amd.serving import _projector, _reference_projector
+
+    reqs = _mix(dev)
+    for r in reqs:
+        p, q = _projector(r, dev), _reference_projector(r, dev)
+        if r.kind == "fan2d":
+            assert p.tvals is None and q.tvals is not None and not q._own_untouched()
+        else:
+            assert p.det_shape is not None and q.det_shape is None
+    saved = _C.AUTO_LAYOUT
+    plain, _ = plain_loop(reqs, dev)
+    ref, _ = plain_loop(reqs, dev, reference_kernels=True)
+    assert _C.AUTO_LAYOUT == saved
+    for r in reqs:
+        a, b = plain[r.rid][0].cpu().numpy(), ref[r.rid][0].cpu().numpy()
+        assert rel_err(b, a) < 1e-4, r.rid
