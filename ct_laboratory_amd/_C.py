@@ -20,8 +20,8 @@ builders' order), the layout is recognised once per ray set and used: the
 forward gets the detector-tiled schedule (same results) and the back
 projection runs on LDS bricks instead of one global atomic per segment (same
 segments, fixed-point sums).  Recognising a layout reads the rays and
-synchronises once; a brick plan is cached per (ray set, grid, M_inv and b
-values).  Cache entries hold weak references to the ray tensors and are used
+synchronises once; a brick plan and its filed unit lists are cached per (ray
+set, grid, M_inv and b values).  Cache entries hold weak references to the ray tensors and are used
 only while those references return the very tensors of the call, unwritten
 (version counters) and at the same storage address, so a new ray set that the
 caching allocator places in a freed block never meets an old entry.  The
@@ -794,10 +794,18 @@ def brick_plan(frames, det_shape, shape):
 
 
 def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH,
-                          out=None, accumulate: bool = False, workspace=None):
+                          out=None, accumulate: bool = False, workspace=None, lists=None):
     """[B, n_x, n_y, n_z] back projection by LDS-resident bricks (same segments as
     back_project_3d_on_the_fly_cuda, no global atomics).  ``plan`` comes from
-    brick_plan(cone_frames(...))."""
+    brick_plan(cone_frames(...)).  With filed lists of the plan (``lists``, or
+    those brick_filed_lists registered for it when ``lists`` is None) the
+    bricks take their rays from the lists; ``lists=False`` files the rays of
+    the shadow rectangles in the call.  The result is the same bit for bit."""
+    if lists is None:
+        lists = filed_lists_of(plan, src, dst, det_shape, (n_x, n_y, n_z))
+    if lists:
+        return back_project_3d_brick_filed(sino, src, dst, det_shape, plan, lists, M_inv, b, n_x, n_y, n_z,
+                                           mode=mode, out=out, accumulate=accumulate, workspace=workspace)
     name = "back_project_3d_brick"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
@@ -819,6 +827,110 @@ def back_project_3d_brick(sino, src, dst, det_shape, plan, M_inv, b, n_x, n_y, n
         N.call("ctlab_back_project_3d_brick", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
                N.ptr(plan), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y), int(n_z), mode, int(bool(accumulate)),
                N.ptr(out), N.ptr(workspace), N.stream_of(sino))
+    return out
+
+
+class FiledLists:
+    """Filed unit lists of a brick plan: offsets int64 [n_slot + 1] (exclusive
+    prefix sums over (unit, length class) slots), ray ids uint32 [total] (as
+    int32 storage), and the launch orders int32 of the units (brick, half of
+    the views) and of the bricks, longest list first, for the plan, rays,
+    detector layout and grid they were built from."""
+
+    def __init__(self, offsets, ids, plan, src, dst, det_shape, shape):
+        self.offsets, self.ids = offsets, ids
+        unit = offsets[FILED_CLASSES::FILED_CLASSES] - offsets[:-1:FILED_CLASSES]
+        self.unit_order = torch.argsort(unit, descending=True, stable=True).to(torch.int32)
+        self.brick_order = torch.argsort(unit.view(-1, 2).sum(1), descending=True, stable=True).to(torch.int32)
+        self._refs = tuple((weakref.ref(t), t._version) for t in (plan, src, dst))
+        self.det_shape = tuple(int(x) for x in det_shape)
+        self.shape = tuple(int(x) for x in shape)
+
+    def matches(self, plan, src, dst, det_shape, shape) -> bool:
+        return (all(r() is t and t._version == v for (r, v), t in zip(self._refs, (plan, src, dst))) and
+                self.det_shape == tuple(int(x) for x in det_shape) and
+                self.shape == tuple(int(x) for x in shape))
+
+
+FILED_CLASSES = 32  # length classes per unit in the filed lists
+
+
+def brick_filed_lists(src, dst, det_shape, plan, M_inv, b, shape, *, max_bytes=None):
+    """Files the rays of every work unit of a brick plan once (two passes and one
+    synchronisation to size the id list): for each (brick, half of the views)
+    the rays of its shadow rectangles that pass through its widened cells,
+    longest estimated first.  Depends only on the geometry.  The lists are
+    registered for ``plan``: back_project_3d_brick then uses them for these
+    rays.  Returns None, registering nothing, when the ids would take more than
+    ``max_bytes``."""
+    name = "brick_filed_lists"
+    _check_f32(name, src, dst, M_inv, b)
+    n_ray = _rays(name, src, dst, 3)
+    rows, cols = (int(x) for x in det_shape)
+    _require(n_ray % (rows * cols) == 0, f"{name}: n_ray is not a multiple of rows*cols")
+    n_view = n_ray // (rows * cols)
+    nx, ny, nz = (int(x) for x in shape)
+    L = N.load()
+    need = L.ctlab_brick_plan_ints_3d(nx, ny, nz, n_view)
+    _require(isinstance(plan, torch.Tensor) and plan.dtype == torch.int32 and plan.numel() == need and plan.is_cuda,
+             f"{name}: plan does not match this geometry")
+    slots = int(L.ctlab_brick_filed_slots_3d(nx, ny, nz))
+    dev = src.device
+    counts = torch.empty((slots,), device=dev, dtype=torch.int32)
+    with torch.cuda.device(dev):
+        N.call("ctlab_brick_filed_count_3d", N.ptr(src), N.ptr(dst), n_view, rows, cols, N.ptr(plan), N.ptr(M_inv),
+               N.ptr(b), nx, ny, nz, N.ptr(counts), N.stream_of(src))
+        offsets = torch.zeros((slots + 1,), device=dev, dtype=torch.int64)
+        torch.cumsum(counts.to(torch.int64) & 0xFFFFFFFF, 0, out=offsets[1:])
+        total = int(offsets[-1])
+        _require(total < 2 ** 32, f"{name}: {total} (unit, ray) pairs exceed 2^32 - 1")
+        if max_bytes is not None and 4 * total > max_bytes:
+            return None
+        ids = torch.empty((max(total, 1),), device=dev, dtype=torch.int32)
+        N.call("ctlab_brick_filed_fill_3d", N.ptr(src), N.ptr(dst), n_view, rows, cols, N.ptr(plan), N.ptr(M_inv),
+               N.ptr(b), nx, ny, nz, N.ptr(offsets), N.ptr(ids), total, N.stream_of(src))
+    lists = FiledLists(offsets, ids, plan, src, dst, det_shape, shape)
+    plan._ctlab_filed = lists  # registered on the plan tensor: the lists live as long as the plan
+    return lists
+
+
+def filed_lists_of(plan, src, dst, det_shape, shape):
+    """The filed lists registered for ``plan`` if they were built for these
+    rays, layout and grid (and none of them changed since), else None."""
+    lists = getattr(plan, "_ctlab_filed", None)
+    if lists is not None and lists.matches(plan, src, dst, det_shape, shape):
+        return lists
+    return None
+
+
+def back_project_3d_brick_filed(sino, src, dst, det_shape, plan, lists, M_inv, b, n_x, n_y, n_z, *,
+                                mode: int = N.MODE_TORCH, out=None, accumulate: bool = False, workspace=None):
+    """back_project_3d_brick over the filed unit lists of its plan
+    (brick_filed_lists): the same result bit for bit."""
+    name = "back_project_3d_brick_filed"
+    _check_f32(name, sino, src, dst, M_inv, b)
+    batch, n_ray = _sino_dims(name, sino)
+    _require(_rays(name, src, dst, 3) == n_ray, f"{name}: sino and src disagree on n_ray")
+    rows, cols = (int(x) for x in det_shape)
+    _require(n_ray % (rows * cols) == 0, f"{name}: n_ray is not a multiple of rows*cols")
+    n_view = n_ray // (rows * cols)
+    need = N.load().ctlab_brick_plan_ints_3d(int(n_x), int(n_y), int(n_z), n_view)
+    _require(isinstance(plan, torch.Tensor) and plan.dtype == torch.int32 and plan.numel() == need and plan.is_cuda,
+             f"{name}: plan does not match this geometry")
+    _require(isinstance(lists, FiledLists) and lists.matches(plan, src, dst, det_shape, (n_x, n_y, n_z)),
+             f"{name}: the filed lists were built for another plan, ray set, layout or grid")
+    if out is None:
+        accumulate = False
+    out = _out(name, out, (batch, n_x, n_y, n_z), sino.device)
+    if workspace is None:
+        workspace = torch.empty(batch * n_view, device=sino.device, dtype=torch.float32)
+    _require(workspace.dtype == torch.float32 and workspace.is_cuda and workspace.numel() >= batch * n_view,
+             f"{name}: workspace must hold batch * n_view floats")
+    with torch.cuda.device(sino.device):
+        N.call("ctlab_back_project_3d_brick_filed", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_view, rows, cols,
+               N.ptr(plan), N.ptr(lists.offsets), N.ptr(lists.ids),
+               N.ptr(lists.brick_order if accumulate else lists.unit_order), N.ptr(M_inv), N.ptr(b), int(n_x),
+               int(n_y), int(n_z), mode, int(bool(accumulate)), N.ptr(out), N.ptr(workspace), N.stream_of(sino))
     return out
 
 
@@ -1207,6 +1319,9 @@ def _plan_of(src, dst, layout, M_inv, b, shape):
     plan = None
     if frames.shape[0] and float(frames[:, FRAME_FLOATS - 1].float().mean()) >= _MIN_VALID_VIEWS:
         plan = brick_plan(frames, layout, shape)
+        # filed unit lists registered on the plan, unless they would take more than half of the free memory
+        free, _ = torch.cuda.mem_get_info(src.device)
+        brick_filed_lists(src, dst, layout, plan, M_inv, b, shape, max_bytes=free // 2)
     with _CACHE_LOCK:
         _remember(_PLANS, key, (weakref.ref(src), weakref.ref(dst), plan))
     return plan

@@ -1396,3 +1396,445 @@ extern "C" int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch
                            n_ray, V, d, bg, offsets, ids, emax, src, dst, M_inv, b, vmax, accumulate, split);
     return check_launch(fn);
 }
+
+// ===========================================================================
+// Filed unit lists: the shadow variant's filing done once per geometry
+// ===========================================================================
+//
+// Which rays of a brick's shadow rectangles pass through the brick's widened
+// cells, and how many crossings each has there, depends only on the geometry.
+// The shadow variant files them again in every call, chunk by chunk, with a
+// block barrier per chunk.  Here one pass per geometry lists, for every work
+// unit (a brick and one half of the views, the shadow variant's units), the
+// rays that pass the same slab test, grouped by the same quarter-octave
+// estimate of their crossing count, longest class first.  The back projector
+// over these lists (k_back_filed) has each wave take the next 64 rays of its
+// unit from an LDS counter and trace them: no filing, no item decoding and no
+// barrier until the unit is done.  Each unit adds up the segments of the same
+// rays as the shadow variant's unit, in 64-bit fixed point with the same
+// scale, so the output is the same bit for bit.
+
+namespace ctlab {
+
+constexpr int kFileThreads = 256;
+constexpr int kFileHalves = 2;     // units per brick: the first and the second half of the views
+constexpr int kFileWaveQueue = 128;  // per-wave LDS queue of rays with a nonzero value
+// Back projector workgroups: 12 waves, three per SIMD, two workgroups per CU
+// (6 waves per SIMD, 80 VGPRs each).  Wave counts that are not a multiple of
+// four leave the SIMDs unevenly loaded; measured on the 256^3 / 720-view
+// cone beam: 768 threads 29.4-30.9 ms, 1024 (64 VGPRs, more spills) 33.9-34.8,
+// 512 31.5, 640 41.4, 896 39.4.
+constexpr int kFiledThreads = 768;
+
+// Largest midpoint error bound over all views (INF when one is not a finite
+// number), times the 2^-10 slack of brick_unit: the bound the shadow variant
+// widens its bricks by.
+__device__ __forceinline__ float filed_brick_e(const float* __restrict__ view_e, int n_view, float* red) {
+    float emax = 0.0f;
+    for (int v = threadIdx.x; v < n_view; v += blockDim.x) {
+        const float ev = view_e[v];
+        emax = fmaxf(emax, (ev >= 0.0f && ev <= 3.0e38f) ? ev : __builtin_huge_valf());
+    }
+    return block_max(emax, red) * 1.0009765625f;
+}
+
+// Pass 1 (FILL false) counts the rays of every unit per length class into
+// counts[unit * kBuckets + (kBuckets - 1 - class)]; pass 2 (FILL true) writes
+// each ray id at offsets[that index] + a cursor, offsets being the exclusive
+// prefix sums of the counts.  One block per unit; items are decoded as
+// brick_items decodes them (plan entries and frames of kStage views staged in
+// LDS per chunk).
+template <bool FILL>
+__global__ __launch_bounds__(kFileThreads) void k_file_units(
+    const float* __restrict__ src, const float* __restrict__ dst, Dims3 dims, BrickGrid bg, int n_view, int rows,
+    int cols, const int32_t* __restrict__ plan, const uint32_t* __restrict__ totals,
+    const float* __restrict__ view_e, const float* __restrict__ frames, const float* __restrict__ minv,
+    const float* __restrict__ b, uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+    uint32_t* __restrict__ ids, int64_t n_ids) {
+    __shared__ uint32_t bcnt[kBuckets];
+    __shared__ int4 s_plan[kStage + 1];
+    __shared__ float s_frame[kStage * kFrameFloats];
+    __shared__ float red[kFileThreads / 64];
+    const int tid = threadIdx.x;
+    const int unit = blockIdx.x, bid = unit / kFileHalves, part = unit % kFileHalves;
+    const Window<3> win = brick_window(dims, bg, bid);
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
+    const uint32_t total = totals[bid];
+    const int v_lo = (int)((int64_t)part * n_view / kFileHalves);
+    const int v_hi = (int)((int64_t)(part + 1) * n_view / kFileHalves);
+    const uint32_t item_lo = (v_lo < n_view) ? (uint32_t)p[(int64_t)v_lo * kPlanInts + 3] : total;
+    const uint32_t item_hi = (v_hi < n_view) ? (uint32_t)p[(int64_t)v_hi * kPlanInts + 3] : total;
+    const float e_brick = filed_brick_e(view_e, n_view, red);
+    const float m = ((e_brick <= 8.0f) ? e_brick : 8.0f) + 0.05f;
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = (float)win.lo[a] - 0.5f - m;
+        bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
+    }
+    if (tid < kBuckets)
+        bcnt[tid] = FILL ? (uint32_t)offsets[(int64_t)unit * kBuckets + (kBuckets - 1 - tid)] : 0u;
+    int vs = 0;
+    {
+        int lo = 0, hi = n_view;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)p[mid * kPlanInts + 3] <= item_lo) lo = mid; else hi = mid;
+        }
+        vs = lo;
+    }
+    constexpr int kFileChunk = 4 * kFileThreads;
+    uint32_t c0 = item_lo;
+    while (c0 < item_hi) {
+        if (tid <= kStage) {
+            const int v = vs + tid;
+            int4 e = make_int4(0, 0, 1, (int)total);  // past the last view: prefix = total
+            if (v < n_view) e = *reinterpret_cast<const int4*>(p + (int64_t)v * kPlanInts);
+            s_plan[tid] = e;
+            if (tid < kStage && v < n_view) {
+                const float* __restrict__ f = frames + (int64_t)v * kFrameFloats;
+#pragma unroll
+                for (int k = 0; k < kFrameFloats; ++k) s_frame[tid * kFrameFloats + k] = f[k];
+            }
+        }
+        __syncthreads();
+        const uint32_t lim = (uint32_t)s_plan[kStage].w;  // first item beyond the staged views
+        const uint32_t c1 = min(c0 + (uint32_t)kFileChunk, min(lim, item_hi));
+        for (uint32_t item = c0 + (uint32_t)tid; item < c1; item += kFileThreads) {
+            const int j = staged_view(s_plan, kStage, item);
+            const int4 e = s_plan[j];
+            uint32_t rem;
+            const int r = e.x + (int)div_rect(item - (uint32_t)e.w, (uint32_t)e.z, &rem);
+            const int c = e.y + (int)rem;
+            const uint32_t ray = (uint32_t)(((int64_t)(vs + j) * rows + r) * cols + c);
+            const float* f = s_frame + j * kFrameFloats;
+            const uint32_t est = (f[F_VALID] != 0.0f)
+                                     ? frame_estimate(f, r, c, blo, bhi)
+                                     : segment_estimate<MODE_TORCH>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
+            if (est) {
+                const uint32_t l2 = 31u - (uint32_t)__clz(est);
+                const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
+                const uint32_t slot = atomicAdd(&bcnt[bk], 1u);
+                if (FILL && (int64_t)slot < n_ids) ids[slot] = ray;
+            }
+        }
+        vs += staged_view(s_plan, kStage + 1, c1);
+        c0 = c1;
+        __syncthreads();
+    }
+    if (!FILL && tid < kBuckets) counts[(int64_t)unit * kBuckets + (kBuckets - 1 - tid)] = bcnt[tid];
+}
+
+// The rays of list[lo, hi) of one unit, traced by the wave that takes them:
+// each wave takes the next 64 ids from *next (LDS), queues the rays with a
+// nonzero value in its own LDS queue (a zero value adds exactly zero) and
+// traces 64 queued rays at a time, so its lanes stay busy on sparse sinograms.
+// Every wave leaves the loop once the list is exhausted and its queue empty.
+template <int MODE, bool DIAG, class Add>
+__device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
+                                            const float* __restrict__ src, const float* __restrict__ dst,
+                                            const uint32_t* __restrict__ list, uint32_t n, float e_brick,
+                                            float s1, float s2, uint32_t* next, uint32_t* wq, Add&& add) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const float e_hint = (e_brick <= 8.0f) ? e_brick : -1.0f;
+    uint32_t pend = 0;  // queued rays (wave-uniform)
+    bool more = true;   // the list has rays left (wave-uniform)
+    while (true) {
+        if (more && pend < 64u) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(next, 64u);
+            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)base, 0));
+            if (base >= n) {
+                more = false;
+            } else {
+                const uint32_t q = base + lane;
+                uint32_t ray = 0u;
+                bool nz = false;
+                if (q < n) {
+                    ray = list[q];
+                    nz = sn[ray] != 0.0f;
+                }
+                const uint64_t mask = __ballot(nz);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                if (nz) wq[pend + below] = ray;  // pend + below < 64 + 64
+                pend += (uint32_t)__popcll(mask);
+                continue;
+            }
+        }
+        if (pend == 0u) break;  // list exhausted and queue empty
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t take = min(pend, 64u);
+        const uint32_t ray = wq[lane];
+        const bool keep = 64u + lane < pend;
+        const uint32_t rest = keep ? wq[64u + lane] : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (keep) wq[lane] = rest;
+        pend -= take;
+        if (lane < take) {
+            const float val = sn[ray] * s1 * s2;
+            float s[3], d[3];
+            for (int a = 0; a < 3; ++a) {
+                s[a] = src[3ull * ray + a];
+                d[a] = dst[3ull * ray + a];
+            }
+            trace_ray_window<3, MODE, DIAG, false>(
+                g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
+        }
+    }
+}
+
+// One workgroup per unit (x) and batch element (y).  split 2: unit = (brick,
+// half); split 1 (accumulate): unit = brick, both halves' lists (contiguous).
+// Value bound, scale, fallback and write-back as brick_unit.
+template <int MODE>
+__global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_back_filed(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    BrickGrid bg, int n_view, const int32_t* __restrict__ plan, const uint32_t* __restrict__ totals,
+    const float* __restrict__ view_e, const float* __restrict__ src, const float* __restrict__ dst,
+    const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ view_max,
+    const int64_t* __restrict__ offsets, const uint32_t* __restrict__ ids, int accumulate, int split,
+    const int32_t* __restrict__ order) {
+    __shared__ unsigned long long acc[kBrickLds];
+    __shared__ uint32_t wq[kFiledThreads / 64 * kFileWaveQueue];
+    __shared__ float s_red[kFiledThreads / 64];
+    __shared__ float s_red2[kFiledThreads / 64];
+    __shared__ uint32_t s_next;
+    const int tid = threadIdx.x;
+    const uint32_t ux = order ? (uint32_t)order[blockIdx.x] : blockIdx.x;
+    if (ux >= gridDim.x) return;  // not a unit (order is meant to be a permutation of them)
+    const int bid = (int)(ux / (uint32_t)split), part = (int)(ux % (uint32_t)split);
+    const int64_t bi = blockIdx.y;
+    const Window<3> win = brick_window(dims, bg, bid);
+    for (int i = tid; i < kBrickLds; i += kFiledThreads) acc[i] = 0ull;
+    if (tid == 0) s_next = 0u;
+
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const float* __restrict__ sn = sino + bi * n_ray;
+    const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
+    const uint32_t total = totals[bid];
+    const int64_t* __restrict__ uo = offsets + (int64_t)bid * kFileHalves * kBuckets;
+    const int64_t lo = (split == 1) ? uo[0] : uo[part * kBuckets];
+    const int64_t hi = (split == 1) ? uo[kFileHalves * kBuckets] : uo[(part + 1) * kBuckets];
+    float* __restrict__ o = out + bi * vol_elems;
+    const int ey = win.ext[1], ez = win.ext[2];
+    const int nvox = win.ext[0] * ey * ez;
+
+    // 1. value bound over the brick's views with a non-empty rectangle and the
+    // largest midpoint error bound over all views (as brick_unit)
+    float vmax = 0.0f, emax = 0.0f;
+    bool nonfinite = false;
+    {
+        const float* __restrict__ vm = view_max + bi * n_view;
+        for (int v = tid; v < n_view; v += kFiledThreads) {
+            if (p[(int64_t)v * kPlanInts + 2] > 0) {
+                const float m = vm[v];
+                nonfinite |= !(m <= 3.0e38f);
+                vmax = fmaxf(vmax, m);
+            }
+            const float ev = view_e[v];
+            emax = fmaxf(emax, (ev >= 0.0f && ev <= 3.0e38f) ? ev : __builtin_huge_valf());
+        }
+    }
+    vmax = nonfinite ? __builtin_huge_valf() : vmax;
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        vmax = fmaxf(vmax, __shfl_xor(vmax, o2));
+        emax = fmaxf(emax, __shfl_xor(emax, o2));
+    }
+    if (tid % 64 == 0) {
+        s_red[tid / 64] = vmax;
+        s_red2[tid / 64] = emax;
+    }
+    __syncthreads();
+    vmax = 0.0f;
+    emax = 0.0f;
+    for (int k = 0; k < kFiledThreads / 64; ++k) {
+        vmax = fmaxf(vmax, s_red[k]);
+        emax = fmaxf(emax, s_red2[k]);
+    }
+    vmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmax)));
+    emax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(emax)));
+    const float e_brick = emax * 1.0009765625f;
+
+    // 2. scale (as brick_unit: the brick's total item count bounds its rays)
+    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
+    const bool fixed = g.finite && cmax <= 1.0e38;
+    const double bound = cmax * (double)(total > 0 ? total : 1);
+    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;
+    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;
+    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
+    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);
+    uint32_t* my_q = wq + (tid / 64) * kFileWaveQueue;
+    const uint32_t* list = ids + lo;
+    const uint32_t n = (uint32_t)(hi - lo);
+
+    // 3. trace and accumulate
+    if (vmax != 0.0f) {
+        if (fixed) {
+            auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
+            if (g.diag)
+                filed_items<MODE, true>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
+            else
+                filed_items<MODE, false>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
+        } else {
+            if (split == 1 && !accumulate) {
+                for (int i = tid; i < nvox; i += kFiledThreads) {
+                    const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                    o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] =
+                        0.0f;
+                }
+            }
+            __syncthreads();
+            auto add = [&](uint32_t li, float c) {
+                const uint32_t r = li % kStrideX;
+                const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
+                const int64_t gi =
+                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+                unsafeAtomicAdd(o + gi, c);
+            };
+            if (g.diag)
+                filed_items<MODE, true>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
+            else
+                filed_items<MODE, false>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
+        }
+    } else if (!fixed && !accumulate && split == 1) {
+        for (int i = tid; i < nvox; i += kFiledThreads) {
+            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+            o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
+        }
+    }
+    if (!fixed || (split > 1 && vmax == 0.0f)) return;
+    __syncthreads();
+
+    // 4. write-back (as brick_unit)
+    const double inv = ldexp(1.0, -F);
+    for (int i = tid; i < nvox; i += kFiledThreads) {
+        const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+        const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
+        if (split > 1)
+            add_f32_ieee(o + gi, v);
+        else
+            o[gi] = accumulate ? o[gi] + v : v;
+    }
+}
+
+}  // namespace ctlab
+
+namespace ctlab {
+
+// Checks shared by the filed-list entry points; fills the plan's sections.
+struct PlanParts {
+    BrickGrid bg;
+    int64_t nb, V;
+    const uint32_t* totals;
+    const float* view_e;
+    const float* frames;
+};
+
+static int filed_check(const char* fn, int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
+                       const float* M_inv, const float* b, const int64_t* n, PlanParts* pp) {
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(n_view >= 1 && det_rows >= 1 && det_cols >= 1 &&
+                      n_view * det_rows * det_cols <= 0xffffffffLL - 4096,
+                  "%s: bad detector layout (n_view*rows*cols must be below 2^32 - 4096)", fn);
+    CTLAB_REQUIRE(plan && M_inv && b, "%s: null pointer", fn);
+    pp->bg = brick_grid(n);
+    pp->nb = (int64_t)pp->bg.nbx * pp->bg.nby * pp->bg.nbz;
+    pp->V = n[0] * n[1] * n[2];
+    pp->totals = reinterpret_cast<const uint32_t*>(plan + pp->nb * n_view * kPlanInts);
+    pp->view_e = reinterpret_cast<const float*>(plan + pp->nb * (n_view * kPlanInts + 1));
+    pp->frames = pp->view_e + n_view;
+    return CTLAB_OK;
+}
+
+}  // namespace ctlab
+
+extern "C" int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z) {
+    using namespace ctlab;
+    if (n_x < 1 || n_y < 1 || n_z < 1) return -1;
+    const int64_t n[3] = {n_x, n_y, n_z};
+    const BrickGrid bg = brick_grid(n);
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * kFileHalves * kBuckets;
+}
+
+extern "C" int ctlab_brick_filed_count_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
+                                          int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b,
+                                          int64_t n_x, int64_t n_y, int64_t n_z, uint32_t* counts,
+                                          ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "brick_filed_count_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    PlanParts pp;
+    int st = filed_check(fn, n_view, det_rows, det_cols, plan, M_inv, b, n, &pp);
+    if (st) return st;
+    CTLAB_REQUIRE(src && dst && counts, "%s: null pointer", fn);
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    hipLaunchKernelGGL(k_file_units<false>, dim3((unsigned)(pp.nb * kFileHalves)), dim3(kFileThreads), 0,
+                       as_stream(stream), src, dst, d, pp.bg, (int)n_view, (int)det_rows, (int)det_cols, plan,
+                       pp.totals, pp.view_e, pp.frames, M_inv, b, counts, nullptr, nullptr, (int64_t)0);
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_brick_filed_fill_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
+                                         int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b,
+                                         int64_t n_x, int64_t n_y, int64_t n_z, const int64_t* offsets,
+                                         uint32_t* ids, int64_t n_ids, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "brick_filed_fill_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    PlanParts pp;
+    int st = filed_check(fn, n_view, det_rows, det_cols, plan, M_inv, b, n, &pp);
+    if (st) return st;
+    CTLAB_REQUIRE(src && dst && offsets && ids && n_ids >= 0, "%s: null pointer", fn);
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    hipLaunchKernelGGL(k_file_units<true>, dim3((unsigned)(pp.nb * kFileHalves)), dim3(kFileThreads), 0,
+                       as_stream(stream), src, dst, d, pp.bg, (int)n_view, (int)det_rows, (int)det_cols, plan,
+                       pp.totals, pp.view_e, pp.frames, M_inv, b, nullptr, offsets, ids, n_ids);
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batch, const float* src,
+                                                 const float* dst, int64_t n_view, int64_t det_rows, int64_t det_cols,
+                                                 const int32_t* plan, const int64_t* offsets, const uint32_t* ids,
+                                                 const int32_t* order, const float* M_inv, const float* b,
+                                                 int64_t n_x, int64_t n_y,
+                                                 int64_t n_z, int mode, int accumulate, float* out, float* view_max,
+                                                 ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "back_project_3d_brick_filed";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    PlanParts pp;
+    int st = filed_check(fn, n_view, det_rows, det_cols, plan, M_inv, b, n, &pp);
+    if (st) return st;
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range", fn, (long long)batch);
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(out && sino && src && dst && offsets && ids && view_max, "%s: null pointer", fn);
+    const int64_t n_ray = n_view * det_rows * det_cols;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    const int split = accumulate ? 1 : kFileHalves;
+    hipLaunchKernelGGL(k_view_absmax, dim3((unsigned)n_view, (unsigned)batch), dim3(256), 0, as_stream(stream), sino,
+                       det_rows * det_cols, (int)n_view, view_max);
+    st = check_launch(fn);
+    if (st) return st;
+    if (split > 1) {
+        st = fill_f32(out, batch * pp.V, 0.0f, as_stream(stream));
+        if (st) return st;
+    }
+    dim3 grid((unsigned)(pp.nb * split), (unsigned)batch);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL(k_back_filed<MODE_TORCH>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out, n_ray,
+                           pp.V, d, pp.bg, (int)n_view, plan, pp.totals, pp.view_e, src, dst, M_inv, b, view_max,
+                           offsets, ids, accumulate, split, order);
+    else
+        hipLaunchKernelGGL(k_back_filed<MODE_OTF>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out, n_ray,
+                           pp.V, d, pp.bg, (int)n_view, plan, pp.totals, pp.view_e, src, dst, M_inv, b, view_max,
+                           offsets, ids, accumulate, split, order);
+    return check_launch(fn);
+}

@@ -232,8 +232,9 @@ def back_project_3d_on_the_fly_hip(sinogram, src, dst, M, b, n_x, n_y, n_z, *, s
 
 class ConeBeamPlan:
     """Geometry-only data of the brick back projector for rays ordered
-    [view][rows][cols] with one source per view: the per-view detector frames
-    and the brick plan.  Built once per geometry on the rays' device."""
+    [view][rows][cols] with one source per view: the per-view detector frames,
+    the brick plan and its filed unit lists.  Built once per geometry on the
+    rays' device."""
 
     # Plans above this size fall back to the atomic back projector.
     max_plan_bytes = 1 << 30
@@ -250,11 +251,17 @@ class ConeBeamPlan:
         ints = N.load().ctlab_brick_plan_ints_3d(*self.shape, self.n_view)
         self.plan = None
         self.frames = None
+        self.lists = None
         if ints <= 0 or 4 * ints > self.max_plan_bytes or n_ray > 0xFFFFFFFF - 4096:
             return
         M_inv = _prepare_M_inv(M)
         self.frames = _C.cone_frames(_f32c(src), _f32c(dst), self.det_shape, M_inv, _f32c(b), self.shape)
         self.plan = _C.brick_plan(self.frames, self.det_shape, self.shape)
+        # the plan's filed unit lists (about 23 bytes per ray at 256^3), registered for the plan, unless
+        # they would take more than half of the free device memory
+        free, _ = torch.cuda.mem_get_info(src.device)
+        self.lists = _C.brick_filed_lists(_f32c(src), _f32c(dst), self.det_shape, self.plan, M_inv, _f32c(b),
+                                          self.shape, max_bytes=free // 2)
 
     @property
     def usable(self) -> bool:

@@ -334,6 +334,43 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
                                 const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
                                 int accumulate, float* out, float* view_max, ctlab_stream_t stream);
 
+/* Filed unit lists of a brick plan (no reference counterpart: they let the
+ * brick back projection above, the alternative schedule of
+ * _C.back_project_3d_on_the_fly_cuda, bindings.cpp:101-103, skip its per-call
+ * filing).  A work unit is one brick and one half of the views (views
+ * [0, n_view/2) and [n_view/2, n_view)); its list holds the rays of its
+ * shadow rectangles that pass the brick kernel's slab test, grouped into 32
+ * classes by estimated crossing count, longest class first.  Slot
+ * (unit * 32 + j) holds class 31 - j of unit = brick * 2 + half:
+ *   1. ctlab_brick_filed_count_3d writes counts[n_slot] (n_slot from
+ *      ctlab_brick_filed_slots_3d);
+ *   2. the caller forms offsets[n_slot + 1], the exclusive prefix sums of the
+ *      counts (int64; the total must stay below 2^32);
+ *   3. ctlab_brick_filed_fill_3d writes each slot's ray ids to
+ *      ids[offsets[s] .. offsets[s+1]) (order within a slot depends on
+ *      timing; n_ids bounds every write).
+ * The lists depend only on the rays, layout, grid, map and plan: build them
+ * once per plan.  ctlab_back_project_3d_brick_filed then gives the result of
+ * ctlab_back_project_3d_brick bit for bit (the same rays per unit, the same
+ * per-brick scale), each wave taking the next 64 rays of its unit and
+ * skipping rays whose value is zero.  order (or NULL) is the launch order of
+ * the units, a permutation of 0 .. n_brick*2 - 1 (of 0 .. n_brick - 1 when
+ * accumulate != 0, where a brick's two halves form one unit); longest unit
+ * first shortens the launch's tail.  view_max as for the brick kernel. */
+int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z);
+int ctlab_brick_filed_count_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
+                               int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b, int64_t n_x,
+                               int64_t n_y, int64_t n_z, uint32_t* counts, ctlab_stream_t stream);
+int ctlab_brick_filed_fill_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
+                              int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b, int64_t n_x,
+                              int64_t n_y, int64_t n_z, const int64_t* offsets, uint32_t* ids, int64_t n_ids,
+                              ctlab_stream_t stream);
+int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batch, const float* src, const float* dst,
+                                      int64_t n_view, int64_t det_rows, int64_t det_cols, const int32_t* plan,
+                                      const int64_t* offsets, const uint32_t* ids, const int32_t* order,
+                                      const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
+                                      int mode, int accumulate, float* out, float* view_max, ctlab_stream_t stream);
+
 /* Brick back projection of rays in any order (no reference counterpart: an
  * alternative schedule of _C.back_project_3d_on_the_fly_cuda, bindings.cpp:101-103,
  * for rays that are not a stack of flat-panel views).  A binning pass lists, per
