@@ -837,9 +837,9 @@ class FiledLists:
     the views) and of the bricks, longest list first, for the plan, rays,
     detector layout and grid they were built from."""
 
-    def __init__(self, offsets, ids, plan, src, dst, det_shape, shape):
+    def __init__(self, offsets, ids, unit_slots, plan, src, dst, det_shape, shape):
         self.offsets, self.ids = offsets, ids
-        unit = offsets[FILED_CLASSES::FILED_CLASSES] - offsets[:-1:FILED_CLASSES]
+        unit = offsets[unit_slots::unit_slots] - offsets[:-1:unit_slots]
         self.unit_order = torch.argsort(unit, descending=True, stable=True).to(torch.int32)
         self.brick_order = torch.argsort(unit.view(-1, 2).sum(1), descending=True, stable=True).to(torch.int32)
         self._refs = tuple((weakref.ref(t), t._version) for t in (plan, src, dst))
@@ -850,9 +850,6 @@ class FiledLists:
         return (all(r() is t and t._version == v for (r, v), t in zip(self._refs, (plan, src, dst))) and
                 self.det_shape == tuple(int(x) for x in det_shape) and
                 self.shape == tuple(int(x) for x in shape))
-
-
-FILED_CLASSES = 32  # length classes per unit in the filed lists
 
 
 def brick_filed_lists(src, dst, det_shape, plan, M_inv, b, shape, *, max_bytes=None):
@@ -874,7 +871,8 @@ def brick_filed_lists(src, dst, det_shape, plan, M_inv, b, shape, *, max_bytes=N
     need = L.ctlab_brick_plan_ints_3d(nx, ny, nz, n_view)
     _require(isinstance(plan, torch.Tensor) and plan.dtype == torch.int32 and plan.numel() == need and plan.is_cuda,
              f"{name}: plan does not match this geometry")
-    slots = int(L.ctlab_brick_filed_slots_3d(nx, ny, nz))
+    slots = int(L.ctlab_brick_filed_slots_3d(nx, ny, nz, n_view))
+    unit_slots = slots // (2 * int(L.ctlab_brick_count_3d(nx, ny, nz)))
     dev = src.device
     counts = torch.empty((slots,), device=dev, dtype=torch.int32)
     with torch.cuda.device(dev):
@@ -889,7 +887,7 @@ def brick_filed_lists(src, dst, det_shape, plan, M_inv, b, shape, *, max_bytes=N
         ids = torch.empty((max(total, 1),), device=dev, dtype=torch.int32)
         N.call("ctlab_brick_filed_fill_3d", N.ptr(src), N.ptr(dst), n_view, rows, cols, N.ptr(plan), N.ptr(M_inv),
                N.ptr(b), nx, ny, nz, N.ptr(offsets), N.ptr(ids), total, N.stream_of(src))
-    lists = FiledLists(offsets, ids, plan, src, dst, det_shape, shape)
+    lists = FiledLists(offsets, ids, unit_slots, plan, src, dst, det_shape, shape)
     plan._ctlab_filed = lists  # registered on the plan tensor: the lists live as long as the plan
     return lists
 

@@ -81,7 +81,7 @@ _SIGS = {
     "ctlab_brick_bins_fill_3d": [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P, _P, _I64, _P],
     "ctlab_back_project_3d_brick_bins": [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
                                          _P, _P, _P],
-    "ctlab_brick_filed_slots_3d": [_I64, _I64, _I64],
+    "ctlab_brick_filed_slots_3d": [_I64, _I64, _I64, _I64],
     "ctlab_brick_filed_count_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _P, _P],
     "ctlab_brick_filed_fill_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _P, _P, _I64, _P],
     "ctlab_back_project_3d_brick_filed": [_P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64,

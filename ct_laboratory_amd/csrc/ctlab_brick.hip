@@ -1418,6 +1418,23 @@ namespace ctlab {
 
 constexpr int kFileThreads = 256;
 constexpr int kFileHalves = 2;     // units per brick: the first and the second half of the views
+constexpr int kFileMaxChunks = 64;  // view chunks per unit, at most
+
+// A unit's list is cut into chunks of consecutive views (at least 8 views, at
+// most kFileMaxChunks chunks per unit), each chunk's rays grouped by length
+// class.  Waves take the list in order, so the workgroups of the launch move
+// through the views together and read the same rays while they sit in L2.
+// Measured on the 256^3 / 720-view cone beam (dense back projection): one
+// chunk per unit 28.9 ms, 32 views per chunk 27.3, 16 views 26.8, 8 views 26.6.
+static int filed_chunk_views(int64_t n_view) {
+    const int64_t half = (n_view + 1) / 2;
+    return (int)std::max<int64_t>(8, (half + kFileMaxChunks - 1) / kFileMaxChunks);
+}
+
+static int filed_unit_slots(int64_t n_view) {
+    const int64_t half = (n_view + 1) / 2, V = filed_chunk_views(n_view);
+    return (int)((half + V - 1) / V) * kBuckets;
+}
 constexpr int kFileWaveQueue = 128;  // per-wave LDS queue of rays with a nonzero value
 // Back projector workgroups: 12 waves, three per SIMD, two workgroups per CU
 // (6 waves per SIMD, 80 VGPRs each).  Wave counts that are not a multiple of
@@ -1449,9 +1466,9 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
     const float* __restrict__ src, const float* __restrict__ dst, Dims3 dims, BrickGrid bg, int n_view, int rows,
     int cols, const int32_t* __restrict__ plan, const uint32_t* __restrict__ totals,
     const float* __restrict__ view_e, const float* __restrict__ frames, const float* __restrict__ minv,
-    const float* __restrict__ b, uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
-    uint32_t* __restrict__ ids, int64_t n_ids) {
-    __shared__ uint32_t bcnt[kBuckets];
+    const float* __restrict__ b, int chunk_views, int unit_slots, uint32_t* __restrict__ counts,
+    const int64_t* __restrict__ offsets, uint32_t* __restrict__ ids, int64_t n_ids) {
+    __shared__ uint32_t bcnt[kFileMaxChunks * kBuckets];
     __shared__ int4 s_plan[kStage + 1];
     __shared__ float s_frame[kStage * kFrameFloats];
     __shared__ float red[kFileThreads / 64];
@@ -1473,8 +1490,8 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
         blo[a] = (float)win.lo[a] - 0.5f - m;
         bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
     }
-    if (tid < kBuckets)
-        bcnt[tid] = FILL ? (uint32_t)offsets[(int64_t)unit * kBuckets + (kBuckets - 1 - tid)] : 0u;
+    for (int i = tid; i < unit_slots; i += kFileThreads)
+        bcnt[i] = FILL ? (uint32_t)offsets[(int64_t)unit * unit_slots + i] : 0u;
     int vs = 0;
     {
         int lo = 0, hi = n_view;
@@ -1515,7 +1532,8 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
             if (est) {
                 const uint32_t l2 = 31u - (uint32_t)__clz(est);
                 const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
-                const uint32_t slot = atomicAdd(&bcnt[bk], 1u);
+                const int ch = (vs + j - v_lo) / chunk_views;
+                const uint32_t slot = atomicAdd(&bcnt[ch * kBuckets + (kBuckets - 1 - (int)bk)], 1u);
                 if (FILL && (int64_t)slot < n_ids) ids[slot] = ray;
             }
         }
@@ -1523,7 +1541,8 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
         c0 = c1;
         __syncthreads();
     }
-    if (!FILL && tid < kBuckets) counts[(int64_t)unit * kBuckets + (kBuckets - 1 - tid)] = bcnt[tid];
+    if (!FILL)
+        for (int i = tid; i < unit_slots; i += kFileThreads) counts[(int64_t)unit * unit_slots + i] = bcnt[i];
 }
 
 // The rays of list[lo, hi) of one unit, traced by the wave that takes them:
@@ -1598,8 +1617,8 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     BrickGrid bg, int n_view, const int32_t* __restrict__ plan, const uint32_t* __restrict__ totals,
     const float* __restrict__ view_e, const float* __restrict__ src, const float* __restrict__ dst,
     const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ view_max,
-    const int64_t* __restrict__ offsets, const uint32_t* __restrict__ ids, int accumulate, int split,
-    const int32_t* __restrict__ order) {
+    const int64_t* __restrict__ offsets, int unit_slots, const uint32_t* __restrict__ ids, int accumulate,
+    int split, const int32_t* __restrict__ order) {
     __shared__ unsigned long long acc[kBrickLds];
     __shared__ uint32_t wq[kFiledThreads / 64 * kFileWaveQueue];
     __shared__ float s_red[kFiledThreads / 64];
@@ -1618,9 +1637,9 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     const float* __restrict__ sn = sino + bi * n_ray;
     const int32_t* __restrict__ p = plan + (int64_t)bid * n_view * kPlanInts;
     const uint32_t total = totals[bid];
-    const int64_t* __restrict__ uo = offsets + (int64_t)bid * kFileHalves * kBuckets;
-    const int64_t lo = (split == 1) ? uo[0] : uo[part * kBuckets];
-    const int64_t hi = (split == 1) ? uo[kFileHalves * kBuckets] : uo[(part + 1) * kBuckets];
+    const int64_t* __restrict__ uo = offsets + (int64_t)bid * kFileHalves * unit_slots;
+    const int64_t lo = (split == 1) ? uo[0] : uo[part * unit_slots];
+    const int64_t hi = (split == 1) ? uo[kFileHalves * unit_slots] : uo[(part + 1) * unit_slots];
     float* __restrict__ o = out + bi * vol_elems;
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
@@ -1756,13 +1775,14 @@ static int filed_check(const char* fn, int64_t n_view, int64_t det_rows, int64_t
 
 }  // namespace ctlab
 
-extern "C" int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z) {
+extern "C" int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view) {
     using namespace ctlab;
-    if (n_x < 1 || n_y < 1 || n_z < 1) return -1;
+    if (n_x < 1 || n_y < 1 || n_z < 1 || n_view < 1) return -1;
     const int64_t n[3] = {n_x, n_y, n_z};
     const BrickGrid bg = brick_grid(n);
-    return (int64_t)bg.nbx * bg.nby * bg.nbz * kFileHalves * kBuckets;
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * kFileHalves * filed_unit_slots(n_view);
 }
+
 
 extern "C" int ctlab_brick_filed_count_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
                                           int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b,
@@ -1778,7 +1798,8 @@ extern "C" int ctlab_brick_filed_count_3d(const float* src, const float* dst, in
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     hipLaunchKernelGGL(k_file_units<false>, dim3((unsigned)(pp.nb * kFileHalves)), dim3(kFileThreads), 0,
                        as_stream(stream), src, dst, d, pp.bg, (int)n_view, (int)det_rows, (int)det_cols, plan,
-                       pp.totals, pp.view_e, pp.frames, M_inv, b, counts, nullptr, nullptr, (int64_t)0);
+                       pp.totals, pp.view_e, pp.frames, M_inv, b, filed_chunk_views(n_view),
+                       filed_unit_slots(n_view), counts, nullptr, nullptr, (int64_t)0);
     return check_launch(fn);
 }
 
@@ -1796,7 +1817,8 @@ extern "C" int ctlab_brick_filed_fill_3d(const float* src, const float* dst, int
     Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
     hipLaunchKernelGGL(k_file_units<true>, dim3((unsigned)(pp.nb * kFileHalves)), dim3(kFileThreads), 0,
                        as_stream(stream), src, dst, d, pp.bg, (int)n_view, (int)det_rows, (int)det_cols, plan,
-                       pp.totals, pp.view_e, pp.frames, M_inv, b, nullptr, offsets, ids, n_ids);
+                       pp.totals, pp.view_e, pp.frames, M_inv, b, filed_chunk_views(n_view),
+                       filed_unit_slots(n_view), nullptr, offsets, ids, n_ids);
     return check_launch(fn);
 }
 
@@ -1831,10 +1853,10 @@ extern "C" int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batc
     if (mode == CTLAB_MODE_TORCH)
         hipLaunchKernelGGL(k_back_filed<MODE_TORCH>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out, n_ray,
                            pp.V, d, pp.bg, (int)n_view, plan, pp.totals, pp.view_e, src, dst, M_inv, b, view_max,
-                           offsets, ids, accumulate, split, order);
+                           offsets, filed_unit_slots(n_view), ids, accumulate, split, order);
     else
         hipLaunchKernelGGL(k_back_filed<MODE_OTF>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out, n_ray,
                            pp.V, d, pp.bg, (int)n_view, plan, pp.totals, pp.view_e, src, dst, M_inv, b, view_max,
-                           offsets, ids, accumulate, split, order);
+                           offsets, filed_unit_slots(n_view), ids, accumulate, split, order);
     return check_launch(fn);
 }

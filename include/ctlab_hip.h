@@ -339,9 +339,12 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
  * _C.back_project_3d_on_the_fly_cuda, bindings.cpp:101-103, skip its per-call
  * filing).  A work unit is one brick and one half of the views (views
  * [0, n_view/2) and [n_view/2, n_view)); its list holds the rays of its
- * shadow rectangles that pass the brick kernel's slab test, grouped into 32
- * classes by estimated crossing count, longest class first.  Slot
- * (unit * 32 + j) holds class 31 - j of unit = brick * 2 + half:
+ * shadow rectangles that pass the brick kernel's slab test, in chunks of V
+ * consecutive views (V = max(8, ceil(ceil(n_view/2) / 64))), each chunk's
+ * rays grouped into 32 classes by estimated crossing count, longest class
+ * first.  With S = ctlab_brick_filed_slots_3d(...) / (2 * n_brick) slots per
+ * unit, slot (unit * S + c * 32 + j) holds class 31 - j of view chunk c of
+ * unit = brick * 2 + half:
  *   1. ctlab_brick_filed_count_3d writes counts[n_slot] (n_slot from
  *      ctlab_brick_filed_slots_3d);
  *   2. the caller forms offsets[n_slot + 1], the exclusive prefix sums of the
@@ -357,7 +360,7 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
  * the units, a permutation of 0 .. n_brick*2 - 1 (of 0 .. n_brick - 1 when
  * accumulate != 0, where a brick's two halves form one unit); longest unit
  * first shortens the launch's tail.  view_max as for the brick kernel. */
-int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z);
+int64_t ctlab_brick_filed_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_view);
 int ctlab_brick_filed_count_3d(const float* src, const float* dst, int64_t n_view, int64_t det_rows,
                                int64_t det_cols, const int32_t* plan, const float* M_inv, const float* b, int64_t n_x,
                                int64_t n_y, int64_t n_z, uint32_t* counts, ctlab_stream_t stream);
