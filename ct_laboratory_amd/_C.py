@@ -938,18 +938,24 @@ def back_project_3d_brick_filed(sino, src, dst, det_shape, plan, lists, M_inv, b
 
 
 class BrickBins:
-    """Per-brick ray lists of a ray set: offsets uint32 [n_brick + 1] (as int32
-    storage), ids uint32 [total] (as int32), emax float32 [1] (the largest
-    midpoint error bound), and the grid and rule they were built for."""
+    """Filed brick lists of a ray set: slot_offsets int64 [n_slot + 1] (the start
+    of every (brick, ray parity, id chunk, length class) slot), ids uint32
+    [total] (as int32 storage, 0xffffffff: no ray), emax float32 [1] (the
+    largest midpoint error bound), the launch orders int32 of the units (brick,
+    parity) and of the bricks, longest list first, and the grid and rule they
+    were built for."""
 
-    def __init__(self, offsets, ids, emax, shape, mode, n_ray):
-        self.offsets, self.ids, self.emax = offsets, ids, emax
+    def __init__(self, slot_offsets, ids, emax, shape, mode, n_ray, unit_slots):
+        self.slot_offsets, self.ids, self.emax = slot_offsets, ids, emax
         self.shape, self.mode, self.n_ray = tuple(int(x) for x in shape), int(mode), int(n_ray)
+        unit = slot_offsets[unit_slots::unit_slots] - slot_offsets[:-1:unit_slots]
+        self.unit_order = torch.argsort(unit, descending=True, stable=True).to(torch.int32)
+        self.brick_order = torch.argsort(unit.view(-1, 2).sum(1), descending=True, stable=True).to(torch.int32)
 
 
 def brick_bins(src, dst, M_inv, b, shape, *, mode: int = N.MODE_TORCH):
-    """Builds the brick lists of a ray set (two passes and one synchronisation
-    to size the id list)."""
+    """Builds the filed brick lists of a ray set (three passes and one
+    synchronisation to size the id list)."""
     name = "brick_bins"
     _check_f32(name, src, dst, M_inv, b)
     n_ray = _rays(name, src, dst, 3)
@@ -959,6 +965,7 @@ def brick_bins(src, dst, M_inv, b, shape, *, mode: int = N.MODE_TORCH):
     L = N.load()
     nbk = int(L.ctlab_brick_count_3d(nx, ny, nz))
     _require(nbk > 0, f"{name}: bad grid")
+    slots = int(L.ctlab_brick_bins_slots_3d(nx, ny, nz, n_ray))
     dev = src.device
     counts = torch.empty((nbk,), device=dev, dtype=torch.int32)
     emax = torch.empty((1,), device=dev, dtype=torch.float32)
@@ -974,14 +981,20 @@ def brick_bins(src, dst, M_inv, b, shape, *, mode: int = N.MODE_TORCH):
         ids = torch.full((max(total, 1),), -1, device=dev, dtype=torch.int32)  # 0xffffffff: no ray
         N.call("ctlab_brick_bins_fill_3d", N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), nx, ny, nz, int(mode),
                N.ptr(cursor), N.ptr(ids), total, N.stream_of(src))
-    return BrickBins(off32, ids, emax, shape, mode, n_ray)
+        del cursor
+        slot_offsets = torch.empty((slots + 1,), device=dev, dtype=torch.int64)
+        filed = torch.full((max(total, 1),), -1, device=dev, dtype=torch.int32)
+        N.call("ctlab_brick_bins_file_3d", N.ptr(src), N.ptr(dst), n_ray, N.ptr(M_inv), N.ptr(b), nx, ny, nz,
+               N.ptr(off32), N.ptr(ids), N.ptr(emax), N.ptr(slot_offsets), N.ptr(filed), N.stream_of(src))
+        del ids
+    return BrickBins(slot_offsets, filed, emax, shape, mode, n_ray, slots // (2 * nbk))
 
 
 def back_project_3d_brick_bins(sino, src, dst, bins, M_inv, b, n_x, n_y, n_z, *, mode: int = N.MODE_TORCH, out=None,
                                accumulate: bool = False, workspace=None):
     """[B, n_x, n_y, n_z] back projection of rays in any order by LDS-resident
-    bricks over the lists ``bins`` (from brick_bins for these rays, grid and
-    rule): the segments of back_project_3d_on_the_fly_cuda, no global atomics."""
+    bricks over the filed lists ``bins`` (from brick_bins for these rays, grid
+    and rule): the segments of back_project_3d_on_the_fly_cuda, no global atomics."""
     name = "back_project_3d_brick_bins"
     _check_f32(name, sino, src, dst, M_inv, b)
     batch, n_ray = _sino_dims(name, sino)
@@ -997,8 +1010,10 @@ def back_project_3d_brick_bins(sino, src, dst, bins, M_inv, b, n_x, n_y, n_z, *,
              f"{name}: workspace must hold batch floats")
     with torch.cuda.device(sino.device):
         N.call("ctlab_back_project_3d_brick_bins", N.ptr(sino), batch, N.ptr(src), N.ptr(dst), n_ray,
-               N.ptr(bins.offsets), N.ptr(bins.ids), N.ptr(bins.emax), N.ptr(M_inv), N.ptr(b), int(n_x), int(n_y),
-               int(n_z), int(mode), int(bool(accumulate)), N.ptr(out), N.ptr(workspace), N.stream_of(sino))
+               N.ptr(bins.slot_offsets), N.ptr(bins.ids), N.ptr(bins.emax),
+               N.ptr(bins.brick_order if accumulate else bins.unit_order), N.ptr(M_inv), N.ptr(b), int(n_x),
+               int(n_y), int(n_z), int(mode), int(bool(accumulate)), N.ptr(out), N.ptr(workspace),
+               N.stream_of(sino))
     return out
 
 
@@ -1151,9 +1166,9 @@ def _bins_of(src, dst, M_inv, b, shape, mode):
         if e is not None and _alive(e[0], src) and _alive(e[1], dst):
             _BINS.move_to_end(key)
             return e[2]
-    # lists of about six (brick, ray) pairs per ray, 4 bytes each, plus the counts
+    # lists of about six (brick, ray) pairs per ray, 4 bytes each, twice while they are refiled
     free, _ = torch.cuda.mem_get_info(src.device)
-    if 32 * int(src.shape[0]) > free // 2:
+    if 64 * int(src.shape[0]) > free // 2:
         return None
     bins = brick_bins(src, dst, M_inv, b, shape, mode=mode)
     with _CACHE_LOCK:

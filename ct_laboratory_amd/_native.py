@@ -79,8 +79,10 @@ _SIGS = {
     "ctlab_brick_count_3d": [_I64, _I64, _I64],
     "ctlab_brick_bins_count_3d": [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P, _P, _P],
     "ctlab_brick_bins_fill_3d": [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _INT, _P, _P, _I64, _P],
-    "ctlab_back_project_3d_brick_bins": [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _INT,
-                                         _P, _P, _P],
+    "ctlab_brick_bins_slots_3d": [_I64, _I64, _I64, _I64],
+    "ctlab_brick_bins_file_3d": [_P, _P, _I64, _P, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P],
+    "ctlab_back_project_3d_brick_bins": [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT,
+                                         _INT, _P, _P, _P],
     "ctlab_brick_filed_slots_3d": [_I64, _I64, _I64, _I64],
     "ctlab_brick_filed_count_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _P, _P],
     "ctlab_brick_filed_fill_3d": [_P, _P, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _P, _P, _I64, _P],
@@ -99,7 +101,7 @@ _SIGS = {
 }
 _RESTYPES = {"ctlab_last_error": ctypes.c_char_p, "ctlab_brick_plan_ints_3d": ctypes.c_int64,
              "ctlab_brick_count_3d": ctypes.c_int64, "ctlab_table_brick_count": ctypes.c_int64,
-             "ctlab_brick_filed_slots_3d": ctypes.c_int64,
+             "ctlab_brick_filed_slots_3d": ctypes.c_int64, "ctlab_brick_bins_slots_3d": ctypes.c_int64,
              "ctlab_back_project_2d_tile_units": ctypes.c_int64}
 
 

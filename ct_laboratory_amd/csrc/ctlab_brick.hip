@@ -1122,165 +1122,6 @@ __global__ __launch_bounds__(256) void k_absmax_all(const float* __restrict__ si
     if (threadIdx.x % 64 == 0) atomicMax(max_bits + blockIdx.y, __float_as_uint(m));
 }
 
-// The rays of a brick list, in chunks as brick_items takes the rays of its
-// shadow rectangles: zero-valued rays and rays that miss the widened brick are
-// dropped, the rest filed by estimated crossing count and traced in that order.
-template <int MODE, bool DIAG, class Add>
-__device__ __forceinline__ void list_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
-                                           const float* __restrict__ src, const float* __restrict__ dst,
-                                           const uint32_t* __restrict__ ids, int64_t n_ray, uint32_t item_lo,
-                                           uint32_t item_hi, float e_brick, float s1, float s2, uint16_t* queue,
-                                           uint32_t* bcnt, Add&& add) {
-    const int tid = threadIdx.x;
-    const bool e_ok = e_brick <= 8.0f;
-    const float e_hint = e_ok ? e_brick : -1.0f;
-    const float m = (e_ok ? e_brick : 8.0f) + 0.05f;
-    float blo[3], bhi[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        blo[a] = (float)win.lo[a] - 0.5f - m;
-        bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
-    }
-    for (uint32_t c0 = item_lo; c0 < item_hi; c0 += (uint32_t)kChunk) {
-        const uint32_t c1 = min(c0 + (uint32_t)kChunk, item_hi);
-        if (tid <= kBuckets) bcnt[tid] = 0u;
-        __syncthreads();
-        uint32_t key[kPerThread];
-#pragma unroll
-        for (int k = 0; k < kPerThread; ++k) {
-            key[k] = ~0u;
-            const uint32_t item = c0 + (uint32_t)tid + (uint32_t)(k * kBrickThreads);
-            if (item < c1) {
-                const uint32_t ray = ids[item];  // unfilled slots hold 0xffffffff
-                if ((int64_t)ray < n_ray && sn[ray] != 0.0f) {
-                    const uint32_t est = segment_estimate<MODE>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
-                    if (est) {
-                        const uint32_t l2 = 31u - (uint32_t)__clz(est);
-                        const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
-                        key[k] = (bk << 12) | atomicAdd(&bcnt[bk], 1u);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < kBuckets; ++k) {
-                const uint32_t c = bcnt[k];
-                bcnt[k] = run;
-                run += c;
-            }
-            bcnt[kBuckets] = run;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kPerThread; ++k)
-            if (key[k] != ~0u) queue[bcnt[key[k] >> 12] + (key[k] & 0xfffu)] = (uint16_t)(tid + k * kBrickThreads);
-        __syncthreads();
-        const uint32_t hits = bcnt[kBuckets];
-        for (uint32_t q = tid; q < hits; q += kBrickThreads) {
-            const uint32_t ray = ids[c0 + queue[q]];
-            const float val = sn[ray] * s1 * s2;
-            float s[3], d[3];
-            for (int a = 0; a < 3; ++a) {
-                s[a] = src[3ull * ray + a];
-                d[a] = dst[3ull * ray + a];
-            }
-            trace_ray_window<3, MODE, DIAG, false>(
-                g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
-        }
-        __syncthreads();
-    }
-}
-
-// One work unit of the list kernel: part `part` of brick `bid` (the part's
-// share of the brick's list) for batch element bi.  The value bound is the
-// batch element's largest |value| and the count the list's length; otherwise
-// as brick_unit.
-template <int MODE>
-__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_back_brick_list(
-    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
-    BrickGrid bg, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ids,
-    const float* __restrict__ emax_in, const float* __restrict__ src, const float* __restrict__ dst,
-    const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ vmax_in, int accumulate,
-    int split) {
-    __shared__ unsigned long long acc[kBrickLds];
-    __shared__ __align__(16) uint16_t queue[kChunk];
-    __shared__ uint32_t bcnt[kBuckets + 1];
-    const int tid = threadIdx.x;
-    const int bid = (int)(blockIdx.x / (uint32_t)split), part = (int)(blockIdx.x % (uint32_t)split);
-    const int64_t bi = blockIdx.y;
-    const Window<3> win = brick_window(dims, bg, bid);
-    const uint32_t base = offsets[bid], total = offsets[bid + 1] - base;
-    const uint32_t item_lo = base + (uint32_t)((uint64_t)total * part / split);
-    const uint32_t item_hi = base + (uint32_t)((uint64_t)total * (part + 1) / split);
-    float* __restrict__ o = out + bi * vol_elems;
-    const int ey = win.ext[1], ez = win.ext[2];
-    const int nvox = win.ext[0] * ey * ez;
-    const float vmax = (total > 0) ? vmax_in[bi] : 0.0f;
-    const float emax = emax_in[0];
-    const float e_brick = (emax >= 0.0f && emax <= 3.0e38f) ? emax * 1.0009765625f : __builtin_huge_valf();
-    if (vmax == 0.0f) {
-        if (split == 1 && !accumulate) {  // split > 1: the host zeroed out
-            for (int i = tid; i < nvox; i += kBrickThreads) {
-                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-                o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
-            }
-        }
-        return;
-    }
-    for (int i = tid; i < kBrickLds; i += kBrickThreads) acc[i] = 0ull;
-    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
-    const float* __restrict__ sn = sino + bi * n_ray;
-    // power-of-two scale as in brick_unit, with the list's length as the count
-    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
-    const bool fixed = g.finite && cmax <= 1.0e38;
-    const double bound = cmax * (double)(total > 0 ? total : 1);
-    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;
-    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;
-    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
-    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);
-    __syncthreads();
-    if (fixed) {
-        auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
-        if (g.diag)
-            list_items<MODE, true>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
-        else
-            list_items<MODE, false>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
-    } else {
-        if (split == 1 && !accumulate) {
-            for (int i = tid; i < nvox; i += kBrickThreads) {
-                const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-                o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
-            }
-        }
-        __syncthreads();
-        auto add = [&](uint32_t li, float c) {
-            const uint32_t r = li % kStrideX;
-            const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
-            const int64_t gi =
-                ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-            unsafeAtomicAdd(o + gi, c);
-        };
-        if (g.diag)
-            list_items<MODE, true>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
-        else
-            list_items<MODE, false>(g, win, sn, src, dst, ids, n_ray, item_lo, item_hi, e_brick, s1, s2, queue, bcnt, add);
-        return;
-    }
-    __syncthreads();
-    const double inv = ldexp(1.0, -F);
-    for (int i = tid; i < nvox; i += kBrickThreads) {
-        const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-        const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
-        if (split > 1)
-            add_f32_ieee(o + gi, v);
-        else
-            o[gi] = accumulate ? o[gi] + v : v;
-    }
-}
-
 static int check_grid3(const char* fn, const int64_t* n) {
     int64_t V = 1;
     for (int a = 0; a < 3; ++a) {
@@ -1351,49 +1192,6 @@ extern "C" int ctlab_brick_bins_fill_3d(const float* src, const float* dst, int6
     else
         hipLaunchKernelGGL(k_bin_fill<false>, dim3(blocks), dim3(kBinThreads), 0, as_stream(stream), src, dst, n_ray, d,
                            bg, M_inv, b, mode, cursor, ids, n_ids);
-    return check_launch(fn);
-}
-
-extern "C" int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch, const float* src, const float* dst,
-                                                int64_t n_ray, const uint32_t* offsets, const uint32_t* ids,
-                                                const float* emax, const float* M_inv, const float* b, int64_t n_x,
-                                                int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
-                                                float* vmax, ctlab_stream_t stream) {
-    using namespace ctlab;
-    const char* fn = "back_project_3d_brick_bins";
-    const int64_t n[3] = {n_x, n_y, n_z};
-    int st = check_grid3(fn, n);
-    if (st) return st;
-    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range", fn, (long long)batch);
-    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
-    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
-    CTLAB_REQUIRE(out && offsets && ids && emax && vmax && M_inv && b, "%s: null pointer", fn);
-    CTLAB_REQUIRE(n_ray == 0 || (sino && src && dst), "%s: null pointer", fn);
-    const BrickGrid bg = brick_grid(n);
-    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
-    const int64_t V = n_x * n_y * n_z;
-    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
-    st = fill_f32(vmax, batch, 0.0f, as_stream(stream));
-    if (st) return st;
-    if (n_ray > 0) {
-        const unsigned blocks = (unsigned)std::min<int64_t>((n_ray + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_absmax_all, dim3(blocks, (unsigned)batch), dim3(256), 0, as_stream(stream), sino, n_ray,
-                           reinterpret_cast<unsigned int*>(vmax));
-        st = check_launch(fn);
-        if (st) return st;
-    }
-    const int split = accumulate ? 1 : 2;
-    if (split > 1) {
-        st = fill_f32(out, batch * V, 0.0f, as_stream(stream));
-        if (st) return st;
-    }
-    dim3 grid((unsigned)(nbk * split), (unsigned)batch);
-    if (mode == CTLAB_MODE_TORCH)
-        hipLaunchKernelGGL(k_back_brick_list<MODE_TORCH>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, offsets, ids, emax, src, dst, M_inv, b, vmax, accumulate, split);
-    else
-        hipLaunchKernelGGL(k_back_brick_list<MODE_OTF>, grid, dim3(kBrickThreads), 0, as_stream(stream), sino, out,
-                           n_ray, V, d, bg, offsets, ids, emax, src, dst, M_inv, b, vmax, accumulate, split);
     return check_launch(fn);
 }
 
@@ -1545,7 +1343,7 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
         for (int i = tid; i < unit_slots; i += kFileThreads) counts[(int64_t)unit * unit_slots + i] = bcnt[i];
 }
 
-// The rays of list[lo, hi) of one unit, traced by the wave that takes them:
+// The rays of list[0, n) of one unit, traced by the wave that takes them:
 // each wave takes the next 64 ids from *next (LDS), queues the rays with a
 // nonzero value in its own LDS queue (a zero value adds exactly zero) and
 // traces 64 queued rays at a time, so its lanes stay busy on sparse sinograms.
@@ -1553,8 +1351,9 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
 template <int MODE, bool DIAG, class Add>
 __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
-                                            const uint32_t* __restrict__ list, uint32_t n, float e_brick,
-                                            float s1, float s2, uint32_t* next, uint32_t* wq, Add&& add) {
+                                            const uint32_t* __restrict__ list, uint32_t n, int64_t n_ray,
+                                            float e_brick, float s1, float s2, uint32_t* next, uint32_t* wq,
+                                            Add&& add) {
     const uint32_t lane = threadIdx.x & 63u;
     const float e_hint = (e_brick <= 8.0f) ? e_brick : -1.0f;
     uint32_t pend = 0;  // queued rays (wave-uniform)
@@ -1572,7 +1371,7 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
                 bool nz = false;
                 if (q < n) {
                     ray = list[q];
-                    nz = sn[ray] != 0.0f;
+                    nz = (int64_t)ray < n_ray && sn[ray] != 0.0f;  // ids of n_ray or more: no ray
                 }
                 const uint64_t mask = __ballot(nz);
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -1611,6 +1410,81 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
 // One workgroup per unit (x) and batch element (y).  split 2: unit = (brick,
 // half); split 1 (accumulate): unit = brick, both halves' lists (contiguous).
 // Value bound, scale, fallback and write-back as brick_unit.
+// Steps 2-4 of a filed work unit (after its value bound vmax, its midpoint
+// error bound e_brick and its ray count bound total are known): the power-of-two
+// scale, the traversal of list[0, n) (filed_items) and the write-back, as
+// brick_unit.  acc is the zeroed LDS brick; *next is zero.
+template <int MODE>
+__device__ __forceinline__ void filed_unit_rest(const Grid<3>& g, const Window<3>& win, const Dims3& dims,
+                                                float* __restrict__ o, const float* __restrict__ sn,
+                                                const float* __restrict__ src, const float* __restrict__ dst,
+                                                const uint32_t* __restrict__ list, uint32_t n, int64_t n_ray,
+                                                float vmax, float e_brick, uint32_t total, int accumulate,
+                                                int split, unsigned long long* acc, uint32_t* s_next,
+                                                uint32_t* my_q) {
+    const int tid = threadIdx.x;
+    const int ey = win.ext[1], ez = win.ext[2];
+    const int nvox = win.ext[0] * ey * ez;
+    // 2. scale (as brick_unit: total bounds the number of rays of the brick)
+    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
+    const bool fixed = g.finite && cmax <= 1.0e38;
+    const double bound = cmax * (double)(total > 0 ? total : 1);
+    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;
+    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;
+    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
+    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);
+
+    // 3. trace and accumulate
+    if (vmax != 0.0f) {
+        if (fixed) {
+            auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
+            if (g.diag)
+                filed_items<MODE, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+            else
+                filed_items<MODE, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+        } else {
+            if (split == 1 && !accumulate) {
+                for (int i = tid; i < nvox; i += kFiledThreads) {
+                    const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+                    o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] =
+                        0.0f;
+                }
+            }
+            __syncthreads();
+            auto add = [&](uint32_t li, float c) {
+                const uint32_t r = li % kStrideX;
+                const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
+                const int64_t gi =
+                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+                unsafeAtomicAdd(o + gi, c);
+            };
+            if (g.diag)
+                filed_items<MODE, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+            else
+                filed_items<MODE, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+        }
+    } else if (!fixed && !accumulate && split == 1) {
+        for (int i = tid; i < nvox; i += kFiledThreads) {
+            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+            o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
+        }
+    }
+    if (!fixed || (split > 1 && vmax == 0.0f)) return;
+    __syncthreads();
+
+    // 4. write-back (as brick_unit)
+    const double inv = ldexp(1.0, -F);
+    for (int i = tid; i < nvox; i += kFiledThreads) {
+        const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
+        const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
+        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
+        if (split > 1)
+            add_f32_ieee(o + gi, v);
+        else
+            o[gi] = accumulate ? o[gi] + v : v;
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_back_filed(
     const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
@@ -1641,8 +1515,6 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     const int64_t lo = (split == 1) ? uo[0] : uo[part * unit_slots];
     const int64_t hi = (split == 1) ? uo[kFileHalves * unit_slots] : uo[(part + 1) * unit_slots];
     float* __restrict__ o = out + bi * vol_elems;
-    const int ey = win.ext[1], ez = win.ext[2];
-    const int nvox = win.ext[0] * ey * ez;
 
     // 1. value bound over the brick's views with a non-empty rectangle and the
     // largest midpoint error bound over all views (as brick_unit)
@@ -1680,67 +1552,8 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     emax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(emax)));
     const float e_brick = emax * 1.0009765625f;
 
-    // 2. scale (as brick_unit: the brick's total item count bounds its rays)
-    const double cmax = 2.0 * (double)vmax * voxel_diagonal(g);
-    const bool fixed = g.finite && cmax <= 1.0e38;
-    const double bound = cmax * (double)(total > 0 ? total : 1);
-    const int e = (bound > 0.0) ? ilogb(bound) + 1 : 0;
-    const int ec = (cmax > 0.0) ? ilogb(cmax) + 1 : 0;
-    const int F = fixed ? max(-250, min(min(62 - e, 50 - ec), 250)) : 0;
-    const float s1 = ldexpf(1.0f, F / 2), s2 = ldexpf(1.0f, F - F / 2);
-    uint32_t* my_q = wq + (tid / 64) * kFileWaveQueue;
-    const uint32_t* list = ids + lo;
-    const uint32_t n = (uint32_t)(hi - lo);
-
-    // 3. trace and accumulate
-    if (vmax != 0.0f) {
-        if (fixed) {
-            auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
-            if (g.diag)
-                filed_items<MODE, true>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
-            else
-                filed_items<MODE, false>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
-        } else {
-            if (split == 1 && !accumulate) {
-                for (int i = tid; i < nvox; i += kFiledThreads) {
-                    const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-                    o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] =
-                        0.0f;
-                }
-            }
-            __syncthreads();
-            auto add = [&](uint32_t li, float c) {
-                const uint32_t r = li % kStrideX;
-                const int ii = (int)(li / kStrideX), j = (int)(r / kStrideY), k = (int)(r % kStrideY);
-                const int64_t gi =
-                    ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-                unsafeAtomicAdd(o + gi, c);
-            };
-            if (g.diag)
-                filed_items<MODE, true>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
-            else
-                filed_items<MODE, false>(g, win, sn, src, dst, list, n, e_brick, s1, s2, &s_next, my_q, add);
-        }
-    } else if (!fixed && !accumulate && split == 1) {
-        for (int i = tid; i < nvox; i += kFiledThreads) {
-            const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-            o[((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k)] = 0.0f;
-        }
-    }
-    if (!fixed || (split > 1 && vmax == 0.0f)) return;
-    __syncthreads();
-
-    // 4. write-back (as brick_unit)
-    const double inv = ldexp(1.0, -F);
-    for (int i = tid; i < nvox; i += kFiledThreads) {
-        const int k = i % ez, j = (i / ez) % ey, ii = i / (ez * ey);
-        const int64_t gi = ((int64_t)(win.lo[0] + ii) * dims.n[1] + (win.lo[1] + j)) * dims.n[2] + (win.lo[2] + k);
-        const float v = (float)((double)(long long)acc[ii * kStrideX + j * kStrideY + k] * inv);
-        if (split > 1)
-            add_f32_ieee(o + gi, v);
-        else
-            o[gi] = accumulate ? o[gi] + v : v;
-    }
+    filed_unit_rest<MODE>(g, win, dims, o, sn, src, dst, ids + lo, (uint32_t)(hi - lo), n_ray, vmax, e_brick,
+                          total, accumulate, split, acc, &s_next, wq + (tid / 64) * kFileWaveQueue);
 }
 
 }  // namespace ctlab
@@ -1858,5 +1671,225 @@ extern "C" int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batc
         hipLaunchKernelGGL(k_back_filed<MODE_OTF>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out, n_ray,
                            pp.V, d, pp.bg, (int)n_view, plan, pp.totals, pp.view_e, src, dst, M_inv, b, view_max,
                            offsets, filed_unit_slots(n_view), ids, accumulate, split, order);
+    return check_launch(fn);
+}
+
+// ===========================================================================
+// Brick lists filed for the lean back projector
+// ===========================================================================
+//
+// The brick lists above (rays in any order) are refiled once per ray set the
+// way the plan's lists are: a brick's list becomes two units, the rays of even
+// and of odd id (about the same work each), each cut into chunks of
+// consecutive ray ids (ray sets in spatial order keep neighbouring rays
+// together) and each chunk grouped by the 32 length classes of the rays' slab
+// estimates, longest first.  k_back_filed_bins then traces them like
+// k_back_filed.
+
+namespace ctlab {
+
+constexpr int kBinChunks = 64;  // id chunks per unit
+
+static int64_t bin_chunk_rays(int64_t n_ray) {
+    const int64_t half = (n_ray + 1) / 2;
+    return std::max<int64_t>(1, (half + kBinChunks - 1) / kBinChunks);
+}
+
+static int bin_unit_slots(int64_t n_ray) {
+    const int64_t half = (n_ray + 1) / 2, R = bin_chunk_rays(n_ray);
+    return (int)std::max<int64_t>(1, (half + R - 1) / R) * kBuckets;
+}
+
+// Exclusive scan of v[0, n) in LDS (n <= 16 * 256) by a 256-thread block;
+// returns the total.  Every thread of the block calls it.
+__device__ uint32_t block_scan_256(uint32_t* v, int n, uint32_t* wsum) {
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    const int per = (n + 255) / 256;
+    const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    uint32_t sum = 0;
+    for (int i = i0; i < i1; ++i) sum += v[i];
+    uint32_t x = sum;  // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = x - sum;
+    for (int k = 0; k < w; ++k) run += wsum[k];
+    const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+// One block per brick: the brick's list ids[offsets[b] .. offsets[b+1]) is
+// written to filed[] in the same range, slot by slot, with slot_offsets[b * 2S
+// + k] the start of slot k (unit (b, ray & 1), chunk (ray >> 1) / chunk_rays,
+// class); ids of n_ray or more (never filled) go to the brick's last slot.
+__global__ __launch_bounds__(256) void k_bin_file(const float* __restrict__ src, const float* __restrict__ dst,
+                                                  int64_t n_ray, Dims3 dims, BrickGrid bg,
+                                                  const float* __restrict__ minv, const float* __restrict__ b,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ ids, const float* __restrict__ emax_in,
+                                                  int64_t chunk_rays, int unit_slots,
+                                                  int64_t* __restrict__ slot_offsets, uint32_t* __restrict__ filed) {
+    __shared__ uint32_t hist[2 * kBinChunks * kBuckets];
+    __shared__ uint32_t wsum[4];
+    const int tid = threadIdx.x;
+    const int bk = blockIdx.x, nslot = 2 * unit_slots;
+    const uint32_t base = offsets[bk], cnt = offsets[bk + 1] - base;
+    const Window<3> win = brick_window(dims, bg, bk);
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const float emax = emax_in[0];
+    const float e_brick = (emax >= 0.0f && emax <= 3.0e38f) ? emax * 1.0009765625f : __builtin_huge_valf();
+    const float m = ((e_brick <= 8.0f) ? e_brick : 8.0f) + 0.05f;
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = (float)win.lo[a] - 0.5f - m;
+        bhi[a] = (float)(win.lo[a] + win.ext[a]) - 0.5f + m;
+    }
+    auto key = [&](uint32_t ray) -> int {
+        if ((int64_t)ray >= n_ray) return nslot - 1;
+        const uint32_t est = segment_estimate<MODE_TORCH>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
+        uint32_t bc = 0;
+        if (est) {
+            const uint32_t l2 = 31u - (uint32_t)__clz(est);
+            bc = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
+        }
+        return (int)(ray & 1u) * unit_slots + (int)((int64_t)(ray >> 1) / chunk_rays) * kBuckets +
+               (kBuckets - 1 - (int)bc);
+    };
+    for (int i = tid; i < nslot; i += 256) hist[i] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < cnt; i += 256) atomicAdd(&hist[key(ids[base + i])], 1u);
+    __syncthreads();
+    block_scan_256(hist, nslot, wsum);
+    for (int i = tid; i < nslot; i += 256) {
+        hist[i] += base;
+        slot_offsets[(int64_t)bk * nslot + i] = hist[i];
+    }
+    if (bk == (int)gridDim.x - 1 && tid == 0) slot_offsets[(int64_t)gridDim.x * nslot] = base + cnt;
+    __syncthreads();
+    for (uint32_t i = tid; i < cnt; i += 256) {
+        const uint32_t ray = ids[base + i];
+        filed[atomicAdd(&hist[key(ray)], 1u)] = ray;
+    }
+}
+
+// One workgroup per unit (brick, ray parity; with split 1 both parities, one
+// contiguous range).  Value bound: the batch element's largest |value|; ray
+// count bound: the brick's list length; midpoint error bound: the largest over
+// the ray set.
+template <int MODE>
+__global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_back_filed_bins(
+    const float* __restrict__ sino, float* __restrict__ out, int64_t n_ray, int64_t vol_elems, Dims3 dims,
+    BrickGrid bg, const int64_t* __restrict__ slot_offsets, int unit_slots, const uint32_t* __restrict__ filed,
+    const float* __restrict__ emax_in, const float* __restrict__ src, const float* __restrict__ dst,
+    const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ vmax_in,
+    int accumulate, int split, const int32_t* __restrict__ order) {
+    __shared__ unsigned long long acc[kBrickLds];
+    __shared__ uint32_t wq[kFiledThreads / 64 * kFileWaveQueue];
+    __shared__ uint32_t s_next;
+    const int tid = threadIdx.x;
+    const uint32_t ux = order ? (uint32_t)order[blockIdx.x] : blockIdx.x;
+    if (ux >= gridDim.x) return;  // not a unit (order is meant to be a permutation of them)
+    const int bid = (int)(ux / (uint32_t)split), part = (int)(ux % (uint32_t)split);
+    const int64_t bi = blockIdx.y;
+    const Window<3> win = brick_window(dims, bg, bid);
+    for (int i = tid; i < kBrickLds; i += kFiledThreads) acc[i] = 0ull;
+    if (tid == 0) s_next = 0u;
+    const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
+    const int64_t* __restrict__ uo = slot_offsets + (int64_t)bid * 2 * unit_slots;
+    const int64_t lo = (split == 1) ? uo[0] : uo[part * unit_slots];
+    const int64_t hi = (split == 1) ? uo[2 * unit_slots] : uo[(part + 1) * unit_slots];
+    const uint32_t total = (uint32_t)(uo[2 * unit_slots] - uo[0]);
+    const float vmax = (total > 0) ? vmax_in[bi] : 0.0f;
+    const float emax = emax_in[0];
+    const float e_brick = (emax >= 0.0f && emax <= 3.0e38f) ? emax * 1.0009765625f : __builtin_huge_valf();
+    __syncthreads();
+    filed_unit_rest<MODE>(g, win, dims, out + bi * vol_elems, sino + bi * n_ray, src, dst, filed + lo,
+                          (uint32_t)(hi - lo), n_ray, vmax, e_brick, total, accumulate, split, acc, &s_next,
+                          wq + (tid / 64) * kFileWaveQueue);
+}
+
+}  // namespace ctlab
+
+extern "C" int64_t ctlab_brick_bins_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_ray) {
+    using namespace ctlab;
+    if (n_x < 1 || n_y < 1 || n_z < 1 || n_ray < 0) return -1;
+    const int64_t n[3] = {n_x, n_y, n_z};
+    const BrickGrid bg = brick_grid(n);
+    return (int64_t)bg.nbx * bg.nby * bg.nbz * 2 * bin_unit_slots(n_ray);
+}
+
+extern "C" int ctlab_brick_bins_file_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv,
+                                        const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
+                                        const uint32_t* offsets, const uint32_t* ids, const float* emax,
+                                        int64_t* slot_offsets, uint32_t* filed, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "brick_bins_file_3d";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
+    CTLAB_REQUIRE(offsets && ids && emax && slot_offsets && filed && M_inv && b && (n_ray == 0 || (src && dst)),
+                  "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    hipLaunchKernelGGL(k_bin_file, dim3((unsigned)nbk), dim3(256), 0, as_stream(stream), src, dst, n_ray, d, bg, M_inv,
+                       b, offsets, ids, emax, bin_chunk_rays(n_ray), bin_unit_slots(n_ray), slot_offsets, filed);
+    return check_launch(fn);
+}
+
+extern "C" int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch, const float* src, const float* dst,
+                                                int64_t n_ray, const int64_t* slot_offsets, const uint32_t* filed,
+                                                const float* emax, const int32_t* order, const float* M_inv,
+                                                const float* b, int64_t n_x, int64_t n_y, int64_t n_z, int mode,
+                                                int accumulate, float* out, float* vmax, ctlab_stream_t stream) {
+    using namespace ctlab;
+    const char* fn = "back_project_3d_brick_bins";
+    const int64_t n[3] = {n_x, n_y, n_z};
+    int st = check_grid3(fn, n);
+    if (st) return st;
+    CTLAB_REQUIRE(batch >= 1 && batch <= 65535, "%s: batch %lld out of range", fn, (long long)batch);
+    CTLAB_REQUIRE(mode == CTLAB_MODE_TORCH || mode == CTLAB_MODE_OTF, "%s: unknown mode %d", fn, mode);
+    CTLAB_REQUIRE(n_ray >= 0 && n_ray <= 0xffffffffLL, "%s: n_ray %lld out of range", fn, (long long)n_ray);
+    CTLAB_REQUIRE(out && slot_offsets && filed && emax && vmax && M_inv && b, "%s: null pointer", fn);
+    CTLAB_REQUIRE(n_ray == 0 || (sino && src && dst), "%s: null pointer", fn);
+    const BrickGrid bg = brick_grid(n);
+    const int64_t nbk = (int64_t)bg.nbx * bg.nby * bg.nbz;
+    const int64_t V = n_x * n_y * n_z;
+    Dims3 d = {{(int)n_x, (int)n_y, (int)n_z}};
+    st = fill_f32(vmax, batch, 0.0f, as_stream(stream));
+    if (st) return st;
+    if (n_ray > 0) {
+        const unsigned blocks = (unsigned)std::min<int64_t>((n_ray + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_absmax_all, dim3(blocks, (unsigned)batch), dim3(256), 0, as_stream(stream), sino, n_ray,
+                           reinterpret_cast<unsigned int*>(vmax));
+        st = check_launch(fn);
+        if (st) return st;
+    }
+    const int split = accumulate ? 1 : 2;
+    if (split > 1) {
+        st = fill_f32(out, batch * V, 0.0f, as_stream(stream));
+        if (st) return st;
+    }
+    dim3 grid((unsigned)(nbk * split), (unsigned)batch);
+    const int slots = bin_unit_slots(n_ray);
+    if (mode == CTLAB_MODE_TORCH)
+        hipLaunchKernelGGL(k_back_filed_bins<MODE_TORCH>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out,
+                           n_ray, V, d, bg, slot_offsets, slots, filed, emax, src, dst, M_inv, b, vmax, accumulate,
+                           split, order);
+    else
+        hipLaunchKernelGGL(k_back_filed_bins<MODE_OTF>, grid, dim3(kFiledThreads), 0, as_stream(stream), sino, out,
+                           n_ray, V, d, bg, slot_offsets, slots, filed, emax, src, dst, M_inv, b, vmax, accumulate,
+                           split, order);
     return check_launch(fn);
 }

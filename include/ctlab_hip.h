@@ -385,25 +385,39 @@ int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batch, const fl
  *      counts, and a copy of offsets[0..n_brick) as cursor;
  *   3. ctlab_brick_bins_fill_3d writes the ray ids of brick k to
  *      ids[offsets[k] .. offsets[k+1]) (in an order that depends on timing;
- *      n_ids, the length of ids, bounds every write, and the back projection
- *      skips ids of n_ray or more, so fill ids with 0xffffffff first).
+ *      n_ids, the length of ids, bounds every write; fill ids with 0xffffffff
+ *      first: ids of n_ray or more are no ray);
+ *   4. ctlab_brick_bins_file_3d refiles each brick's list in place order into
+ *      filed[offsets[k] .. offsets[k+1]): two units per brick (rays of even and
+ *      of odd id), each in chunks of consecutive ids, each chunk grouped by 32
+ *      classes of estimated crossing count, longest first; slot_offsets
+ *      [n_slot + 1] (int64, n_slot from ctlab_brick_bins_slots_3d) holds the
+ *      start of every slot, slot (brick * 2 + parity) * S + chunk * 32 + j with
+ *      S = n_slot / (2 * n_brick).
  * The lists depend only on the rays, grid, map and rule: build them once per ray
  * set.  ctlab_back_project_3d_brick_bins then computes the same segments as
  * ctlab_back_project_3d_otf brick by brick in LDS (64-bit fixed point, no global
- * atomics, results independent of the order of the lists); vmax is caller
- * workspace of batch floats.  n_ray must be below 2^32.  ctlab_brick_count_3d
- * returns n_brick. */
+ * atomics, results independent of the order within the lists), each wave
+ * taking the next 64 rays of its unit; order (or NULL) is the launch order of
+ * the units, a permutation of 0 .. n_brick*2 - 1 (of 0 .. n_brick - 1 when
+ * accumulate != 0); vmax is caller workspace of batch floats.  n_ray must be
+ * below 2^32.  ctlab_brick_count_3d returns n_brick. */
 int64_t ctlab_brick_count_3d(int64_t n_x, int64_t n_y, int64_t n_z);
+int64_t ctlab_brick_bins_slots_3d(int64_t n_x, int64_t n_y, int64_t n_z, int64_t n_ray);
 int ctlab_brick_bins_count_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv, const float* b,
                               int64_t n_x, int64_t n_y, int64_t n_z, int mode, uint32_t* counts, float* emax,
                               ctlab_stream_t stream);
 int ctlab_brick_bins_fill_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv, const float* b,
                              int64_t n_x, int64_t n_y, int64_t n_z, int mode, uint32_t* cursor, uint32_t* ids,
                              int64_t n_ids, ctlab_stream_t stream);
+int ctlab_brick_bins_file_3d(const float* src, const float* dst, int64_t n_ray, const float* M_inv, const float* b,
+                             int64_t n_x, int64_t n_y, int64_t n_z, const uint32_t* offsets, const uint32_t* ids,
+                             const float* emax, int64_t* slot_offsets, uint32_t* filed, ctlab_stream_t stream);
 int ctlab_back_project_3d_brick_bins(const float* sino, int64_t batch, const float* src, const float* dst,
-                                     int64_t n_ray, const uint32_t* offsets, const uint32_t* ids, const float* emax,
-                                     const float* M_inv, const float* b, int64_t n_x, int64_t n_y, int64_t n_z,
-                                     int mode, int accumulate, float* out, float* vmax, ctlab_stream_t stream);
+                                     int64_t n_ray, const int64_t* slot_offsets, const uint32_t* filed,
+                                     const float* emax, const int32_t* order, const float* M_inv, const float* b,
+                                     int64_t n_x, int64_t n_y, int64_t n_z, int mode, int accumulate, float* out,
+                                     float* vmax, ctlab_stream_t stream);
 
 /* Largest batch * n_view of ctlab_back_project_3d_brick_noalloc (256 KiB of
  * per-view maxima, held by each device the library runs on until the process
