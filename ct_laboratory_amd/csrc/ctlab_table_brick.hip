@@ -117,33 +117,72 @@ __global__ __launch_bounds__(kBlock) void k_table_runs(Dims3 dims, BrickGrid bg,
     if (active && cur != 0xffffffffu) close();
 }
 
+// A table block in flight: raw is the 16-byte load, fix the block's entries
+// when it is not wholly inside the table (zero outside it), taken from the
+// table's two edge blocks staged in LDS.  raw is then loaded from a block that
+// is wholly inside (first_full, the first entry of one), so every block costs
+// one global load and the value is selected only when the block is used: a
+// block loaded ahead is not waited for at the load, as it would be if a path
+// with loads of its own wrote into the load's registers.  A table with no
+// whole block (first_full < 0) takes its entries one by one.
+template <typename V>
+struct TableBlock {
+    V raw, fix;
+    bool inside;
+    __device__ __forceinline__ V value() const { return inside ? raw : fix; }
+};
+
 // Block q of 8 uint16 entries: entries [8q - off, 8q - off + 8) of the flat
 // table of n_ent entries, whose first entry sits `off` entries past a 16-byte
-// boundary.  One 16-byte load when the block lies wholly inside the table,
-// else its entries one by one (zero outside; those are never consumed).
-__device__ __forceinline__ uint4 load_block(const uint16_t* __restrict__ u16, int64_t q, int off, int64_t n_ent) {
+// boundary.  edge holds blocks 0 and (n_ent - 1 + off) / 8; blocks past the
+// table are never consumed.
+__device__ __forceinline__ TableBlock<uint4> load_block(const uint16_t* __restrict__ u16, int64_t q, int off,
+                                                        int64_t n_ent, int64_t first_full, const uint4* edge) {
+    TableBlock<uint4> b;
     const int64_t e0 = q * 8 - off;
-    if (e0 >= 0 && e0 + 8 <= n_ent) return *reinterpret_cast<const uint4*>(u16 + e0);
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-    for (int i = 0; i < 8; ++i) {
-        const int64_t e = e0 + i;
-        if (e >= 0 && e < n_ent) w[i >> 1] |= (uint32_t)u16[e] << ((i & 1) * 16);
+    b.inside = e0 >= 0 && e0 + 8 <= n_ent;
+    if (first_full < 0) {  // uniform: a table with no whole block
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        for (int i = 0; i < 8; ++i) {
+            const int64_t e = e0 + i;
+            if (e >= 0 && e < n_ent) w[i >> 1] |= (uint32_t)u16[e] << ((i & 1) * 16);
+        }
+        b.raw = b.fix = make_uint4(w[0], w[1], w[2], w[3]);
+        return b;
     }
-    return make_uint4(w[0], w[1], w[2], w[3]);
+    b.raw = *reinterpret_cast<const uint4*>(u16 + (b.inside ? e0 : first_full));
+    b.fix = (e0 < 0) ? edge[0] : edge[1];
+    return b;
 }
 
 // Block q of 4 float entries: entries [4q - off, 4q - off + 4) of the flat
 // table of n_ent entries, whose first entry sits `off` entries past a 16-byte
 // boundary; loaded as load_block loads uint16 blocks.
-__device__ __forceinline__ float4 load_block4(const float* __restrict__ tv, int64_t q, int off, int64_t n_ent) {
+__device__ __forceinline__ TableBlock<float4> load_block4(const float* __restrict__ tv, int64_t q, int off,
+                                                          int64_t n_ent, int64_t first_full, const float4* edge) {
+    TableBlock<float4> b;
     const int64_t e0 = q * 4 - off;
-    if (e0 >= 0 && e0 + 4 <= n_ent) return *reinterpret_cast<const float4*>(tv + e0);
-    float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int i = 0; i < 4; ++i) {
-        const int64_t e = e0 + i;
-        if (e >= 0 && e < n_ent) w[i] = tv[e];
+    b.inside = e0 >= 0 && e0 + 4 <= n_ent;
+    if (first_full < 0) {
+        float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 4; ++i) {
+            const int64_t e = e0 + i;
+            if (e >= 0 && e < n_ent) w[i] = tv[e];
+        }
+        b.raw = b.fix = make_float4(w[0], w[1], w[2], w[3]);
+        return b;
     }
-    return make_float4(w[0], w[1], w[2], w[3]);
+    b.raw = *reinterpret_cast<const float4*>(tv + (b.inside ? e0 : first_full));
+    b.fix = (e0 < 0) ? edge[0] : edge[1];
+    return b;
+}
+
+// First entry of the first block wholly inside a table of n_ent entries whose
+// first entry sits `off` entries (of per_block per block) past a block
+// boundary, or -1 when there is none.
+__device__ __forceinline__ int64_t first_full_block(int64_t n_ent, int off, int per_block) {
+    const int64_t e = (off == 0) ? 0 : per_block - off;
+    return (e + per_block <= n_ent) ? e : -1;
 }
 
 __device__ __forceinline__ float pick4(const float4& b, int64_t gi) {
@@ -159,7 +198,8 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
                                                  const float* __restrict__ src, const float* __restrict__ dst,
                                                  int64_t n_ray, const ctlab_table_run_t* __restrict__ runs,
                                                  int64_t n_runs, const ctlab_table_unit_t& u, int64_t bi,
-                                                 int u16_off, int f32_off) {
+                                                 int u16_off, int f32_off, const uint4* edge16,
+                                                 const float4* edge32) {
     MidIndex<3, DIAG> mi;
     mi.init(g);
     const float* __restrict__ sn = sino + bi * n_ray;
@@ -170,15 +210,20 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
         if (lx < (uint32_t)kTbX && ly < (uint32_t)kTbY && lz < (uint32_t)kTbZ)
             atomicAdd(&acc[(lx * kTbY + ly) * kTbZ + lz], w);
     };
-    for (int64_t q = threadIdx.x; q < (int64_t)u.n; q += kTbBlock) {
-        const int64_t pos = u.first + q;
-        if (pos >= n_runs) break;
-        const ctlab_table_run_t rr = runs[pos];
+    // A lane's runs are q = tid, tid + kTbBlock, ...: the next run's record is
+    // loaded while the current run is walked, and a run's value, ray and table
+    // loads are all issued before its value is tested.
+    const int64_t n_q = min((int64_t)u.n, n_runs - u.first);
+    int64_t q = threadIdx.x;
+    ctlab_table_run_t nrec = {};
+    if (q < n_q) nrec = runs[u.first + q];
+    for (; q < n_q; q += kTbBlock) {
+        const ctlab_table_run_t rr = nrec;
+        if (q + kTbBlock < n_q) nrec = runs[u.first + q + kTbBlock];
         const int64_t ray = rr.ray;
         const int64_t j0 = rr.seg0, j1 = (int64_t)rr.seg0 + rr.nseg;
         if (ray >= n_ray || j0 < 1 || j1 > K) continue;
         const float val = sn[ray];
-        if (val == 0.0f) continue;
         float s[3], vw[3], p0[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -187,18 +232,26 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
         }
         if (U16) {
             const float st0 = start[ray], sc = scale[ray];
+            // entries read 8 at a time in 16-byte blocks (gi counts entries from the
+            // 16-byte boundary at or before the table's start); the next block is
+            // loaded while the current one is walked
+            const int64_t n_ent = n_ray * K;
+            int64_t gi = ray * K + j0 + u16_off;
+            const int64_t ff = first_full_block(n_ent, u16_off, 8);
+            const TableBlock<uint4> first = load_block(u16, gi >> 3, u16_off, n_ent, ff, edge16);
+            TableBlock<uint4> nxt = load_block(u16, (gi >> 3) + 1, u16_off, n_ent, ff, edge16);
+            if (val == 0.0f) continue;
             // t of entry j0-1: start for entry 0, else start + k * scale (3d.cu:128-129)
             const float t0 = (j0 == 1) ? st0 : __builtin_fmaf((float)rr.k0, sc, st0);
 #pragma unroll
             for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
-            // entries read 8 at a time in 16-byte blocks (gi counts entries from the
-            // 16-byte boundary at or before the table's start)
-            const int64_t n_ent = n_ray * K;
-            int64_t gi = ray * K + j0 + u16_off;
-            uint4 blk = load_block(u16, gi >> 3, u16_off, n_ent);
             uint32_t ks = rr.k0;
+            uint4 blk = first.value();
             for (int64_t j = j0; j < j1; ++j, ++gi) {
-                if ((gi & 7) == 0 && j != j0) blk = load_block(u16, gi >> 3, u16_off, n_ent);
+                if ((gi & 7) == 0 && j != j0) {
+                    blk = nxt.value();
+                    nxt = load_block(u16, (gi >> 3) + 1, u16_off, n_ent, ff, edge16);
+                }
                 const uint32_t w = (gi & 4) ? ((gi & 2) ? blk.w : blk.z) : ((gi & 2) ? blk.y : blk.x);
                 ks += (gi & 1) ? (w >> 16) : (w & 0xffffu);
                 const float t1 = __builtin_fmaf((float)ks, sc, st0);
@@ -211,13 +264,19 @@ __device__ __forceinline__ void table_brick_runs(float* acc, const Grid<3>& g, i
             // entries read 4 at a time in 16-byte blocks, as the uint16 ones
             const int64_t n_ent = n_ray * K;
             int64_t gi = ray * K + j0 - 1 + f32_off;
-            float4 blk = load_block4(tvals, gi >> 2, f32_off, n_ent);
+            const int64_t ff = first_full_block(n_ent, f32_off, 4);
+            float4 blk = load_block4(tvals, gi >> 2, f32_off, n_ent, ff, edge32).value();
+            TableBlock<float4> nxt = load_block4(tvals, (gi >> 2) + 1, f32_off, n_ent, ff, edge32);
+            if (val == 0.0f) continue;
             float t0 = pick4(blk, gi);
 #pragma unroll
             for (int a = 0; a < 3; ++a) p0[a] = s[a] + t0 * vw[a];
             for (int64_t j = j0; j < j1; ++j) {
                 ++gi;
-                if ((gi & 3) == 0) blk = load_block4(tvals, gi >> 2, f32_off, n_ent);
+                if ((gi & 3) == 0) {
+                    blk = nxt.value();
+                    nxt = load_block4(tvals, (gi >> 2) + 1, f32_off, n_ent, ff, edge32);
+                }
                 const float t1 = pick4(blk, gi);
                 if (isinf(t0) || isinf(t1)) {  // end of a sorted row; a finite value restarts the walk
                     t0 = t1;
@@ -253,13 +312,33 @@ __global__ __launch_bounds__(kTbBlock) void k_back_table_brick(
     const int bx = (int)(u.brick / ((uint32_t)bg.nb[2] * (uint32_t)bg.nb[1]));
     const int x0 = bx * kTbX, y0 = by * kTbY, z0 = bz * kTbZ;
     for (int i = threadIdx.x; i < kTbVox; i += kTbBlock) acc[i] = 0.0f;
+    // the table's first and last 16-byte blocks (entries outside the table: 0)
+    __shared__ __align__(16) uint32_t edge[8];
+    {
+        const int64_t n_ent = n_ray * K;
+        const int t = threadIdx.x;
+        if (U16 && t < 16) {
+            const int off = u16_off;
+            const int64_t q = (t < 8) ? 0 : (n_ent - 1 + off) / 8;
+            const int64_t e = q * 8 - off + (t & 7);
+            reinterpret_cast<uint16_t*>(edge)[t] = (e >= 0 && e < n_ent) ? u16[e] : (uint16_t)0;
+        }
+        if (!U16 && t < 8) {
+            const int off = f32_off;
+            const int64_t q = (t < 4) ? 0 : (n_ent - 1 + off) / 4;
+            const int64_t e = q * 4 - off + (t & 3);
+            reinterpret_cast<float*>(edge)[t] = (e >= 0 && e < n_ent) ? tvals[e] : 0.0f;
+        }
+    }
     __syncthreads();
     if (g.diag)
         table_brick_runs<U16, true>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
-                                    n_runs, u, bi, u16_off, f32_off);
+                                    n_runs, u, bi, u16_off, f32_off, reinterpret_cast<const uint4*>(edge),
+                                    reinterpret_cast<const float4*>(edge));
     else
         table_brick_runs<U16, false>(acc, g, x0, y0, z0, sino, tvals, u16, start, scale, K, src, dst, n_ray, runs,
-                                     n_runs, u, bi, u16_off, f32_off);
+                                     n_runs, u, bi, u16_off, f32_off, reinterpret_cast<const uint4*>(edge),
+                                    reinterpret_cast<const float4*>(edge));
     __syncthreads();
     float* __restrict__ o = out + bi * vol_elems;
     for (int i = threadIdx.x; i < kTbVox; i += kTbBlock) {
