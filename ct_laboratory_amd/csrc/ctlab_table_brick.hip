@@ -386,7 +386,9 @@ __global__ __launch_bounds__(kBlock) void k_forward_table_ranged(
     const int lo = s_lo, hi = s_hi;  // block-uniform
     const Grid<3> g = load_grid<3>(dims.n, minv, b, ZERO_THRESH);
     const float* __restrict__ v = vol + bi * vol_elems;
-    float acc = 0.0f;
+    // gathers consumed one segment later (their loads overlap the next
+    // segment); the same chain of fused multiply-adds, fma(0, 0, +0) = +0 first
+    float acc = 0.0f, pv = 0.0f, pl = 0.0f;
     if (lo < hi && g.finite) {
         float s[3] = {}, vw[3] = {}, p0[3] = {};
         float st0 = 0.0f, sc = 0.0f, t0 = 0.0f;
@@ -430,7 +432,11 @@ __global__ __launch_bounds__(kBlock) void k_forward_table_ranged(
                         const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
                         if (!(len >= 1e-9f)) continue;  // 3d.cu:137
                         uint32_t idx;
-                        if (flat_voxel<3>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+                        if (flat_voxel<3>(g, c, &idx)) {
+                            acc = __builtin_fmaf(pv, pl, acc);
+                            pv = v[idx];
+                            pl = len;
+                        }
                     } else {
                         const float t1 = mine[j - c0];
                         if (isinf(t0) || isinf(t1)) {  // end of a sorted row; a finite value restarts the walk
@@ -442,7 +448,11 @@ __global__ __launch_bounds__(kBlock) void k_forward_table_ranged(
                         const float len = segment_mid<3, DIAG>(mi, s, vw, p0, t1, c);
                         t0 = t1;
                         uint32_t idx;
-                        if (len > 0.0f && flat_voxel<3>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+                        if (len > 0.0f && flat_voxel<3>(g, c, &idx)) {
+                            acc = __builtin_fmaf(pv, pl, acc);
+                            pv = v[idx];
+                            pl = len;
+                        }
                     }
                 }
             }
@@ -452,6 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_forward_table_ranged(
         else
             run(std::integral_constant<bool, false>());
     }
+    acc = __builtin_fmaf(pv, pl, acc);
     if (active) out[bi * n_ray + r] = acc;
 }
 

@@ -98,15 +98,23 @@ __global__ __launch_bounds__(kBlock) void k_forward_tvals(const float* __restric
     float s[ND] = {}, d[ND] = {};
     if (active) load_ray_t<ND>(src, dst, r, s, d);
     const float* __restrict__ v = vol + bi * vol_elems;
-    float acc = 0.0f;
+    // Each gathered value is consumed one segment later, so its load is in
+    // flight while the next segment is computed; the chain of fused
+    // multiply-adds is the same (the first, fma(0, 0, +0), leaves +0).
+    float acc = 0.0f, pv = 0.0f, pl = 0.0f;
     auto gather = [&](int64_t, uint32_t, const float* c, float len) {
         uint32_t idx;
-        if (flat_voxel<ND>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+        if (flat_voxel<ND>(g, c, &idx)) {
+            acc = __builtin_fmaf(pv, pl, acc);
+            pv = v[idx];
+            pl = len;
+        }
     };
     if (g.diag)
         walk_table<ND, true>(g, s, d, tvals, K, r0, n_ray, active, st, gather);
     else
         walk_table<ND, false>(g, s, d, tvals, K, r0, n_ray, active, st, gather);
+    acc = __builtin_fmaf(pv, pl, acc);
     if (active) out[bi * n_ray + r] = acc;
 }
 
@@ -224,15 +232,20 @@ __global__ __launch_bounds__(kBlock) void k_forward_u16(const float* __restrict_
         sc = scale[r];
     }
     const float* __restrict__ v = vol + bi * vol_elems;
-    float acc = 0.0f;
+    float acc = 0.0f, pv = 0.0f, pl = 0.0f;  // gathers consumed one segment later (as k_forward_tvals)
     auto gather = [&](int64_t, uint32_t, const float* c, float len) {
         uint32_t idx;
-        if (flat_voxel<3>(g, c, &idx)) acc = __builtin_fmaf(v[idx], len, acc);
+        if (flat_voxel<3>(g, c, &idx)) {
+            acc = __builtin_fmaf(pv, pl, acc);
+            pv = v[idx];
+            pl = len;
+        }
     };
     if (g.diag)
         walk_u16<true>(g, s, d, u16, K, r0, n_ray, active, st0, sc, st, gather);
     else
         walk_u16<false>(g, s, d, u16, K, r0, n_ray, active, st0, sc, st, gather);
+    acc = __builtin_fmaf(pv, pl, acc);
     if (active) out[bi * n_ray + r] = acc;
 }
 
