@@ -1348,7 +1348,7 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
 // nonzero value in its own LDS queue (a zero value adds exactly zero) and
 // traces 64 queued rays at a time, so its lanes stay busy on sparse sinograms.
 // Every wave leaves the loop once the list is exhausted and its queue empty.
-template <int MODE, bool DIAG, class Add>
+template <int MODE, bool DIAG, bool POW2, bool ZERO, class Add>
 __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
                                             const uint32_t* __restrict__ list, uint32_t n, int64_t n_ray,
@@ -1401,7 +1401,7 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
                 s[a] = src[3ull * ray + a];
                 d[a] = dst[3ull * ray + a];
             }
-            trace_ray_window<3, MODE, DIAG, false>(
+            trace_ray_window<3, MODE, DIAG, false, POW2, ZERO>(
                 g, win, s, d, [&](uint32_t li, float len) { add(li, val * len); }, e_hint);
         }
     }
@@ -1437,11 +1437,17 @@ __device__ __forceinline__ void filed_unit_rest(const Grid<3>& g, const Window<3
     // 3. trace and accumulate
     if (vmax != 0.0f) {
         if (fixed) {
+            // finite values: a zero-length segment adds an exact zero, so it is not tested for
             auto add = [&](uint32_t li, float c) { atomicAdd(&acc[li], to_fixed(c)); };
-            if (g.diag)
-                filed_items<MODE, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+            if (MODE == MODE_TORCH && g.pow2)
+                filed_items<MODE, true, MODE == MODE_TORCH, true>(g, win, sn, src, dst, list, n, n_ray, e_brick,
+                                                                   s1, s2, s_next, my_q, add);
+            else if (g.diag)
+                filed_items<MODE, true, false, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next,
+                                                     my_q, add);
             else
-                filed_items<MODE, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+                filed_items<MODE, false, false, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next,
+                                                      my_q, add);
         } else {
             if (split == 1 && !accumulate) {
                 for (int i = tid; i < nvox; i += kFiledThreads) {
@@ -1459,9 +1465,11 @@ __device__ __forceinline__ void filed_unit_rest(const Grid<3>& g, const Window<3
                 unsafeAtomicAdd(o + gi, c);
             };
             if (g.diag)
-                filed_items<MODE, true>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+                filed_items<MODE, true, false, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next,
+                                                      my_q, add);
             else
-                filed_items<MODE, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next, my_q, add);
+                filed_items<MODE, false, false, false>(g, win, sn, src, dst, list, n, n_ray, e_brick, s1, s2, s_next,
+                                                       my_q, add);
         }
     } else if (!fixed && !accumulate && split == 1) {
         for (int i = tid; i < nvox; i += kFiledThreads) {

@@ -119,12 +119,21 @@ __global__ __launch_bounds__(kBlock) void k_forward_otf(const float* __restrict_
         }
         pl = len;
     };
-    if constexpr (MODE == MODE_MARCH)
+    if constexpr (MODE == MODE_MARCH) {
         march_ray_2d(g, s, d, gather);
-    else if (g.diag)
-        trace_ray<ND, MODE, true>(g, s, d, gather);
-    else
-        trace_ray<ND, MODE, false>(g, s, d, gather);
+    } else if constexpr (MODE == MODE_TORCH) {
+        if (g.pow2)
+            trace_ray<ND, MODE, true, true>(g, s, d, gather);
+        else if (g.diag)
+            trace_ray<ND, MODE, true>(g, s, d, gather);
+        else
+            trace_ray<ND, MODE, false>(g, s, d, gather);
+    } else {
+        if (g.diag)
+            trace_ray<ND, MODE, true>(g, s, d, gather);
+        else
+            trace_ray<ND, MODE, false>(g, s, d, gather);
+    }
 #pragma unroll
     for (int k = 0; k < NB; ++k)
         if (b0 + k < batch) out[(int64_t)(b0 + k) * n_ray + r] = __builtin_fmaf(pv[k], pl, acc[k]);

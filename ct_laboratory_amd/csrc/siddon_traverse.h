@@ -45,6 +45,7 @@ struct Grid {
     float minv[ND * ND];
     float b[ND];
     int diag;       // off-diagonal entries of minv are all zero
+    int pow2;       // diag, and every diagonal entry of minv is +-2^k (normal)
     int finite;     // minv and b are finite
     int zero_rule;  // ZERO_THRESH: |dir| < 1e-12 has no crossings; ZERO_EXACT: dir == 0
 };
@@ -66,7 +67,14 @@ __host__ __device__ __forceinline__ Grid<ND> load_grid(const int* n, const float
         if ((i / ND) != (i % ND) && g.minv[i] != 0.0f) diag = 0;
         fin &= (fabsf(g.minv[i]) <= 3.0e38f);
     }
+    int pow2 = diag;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, g.minv[a * ND + a]) & 0x7fffffffu;
+        pow2 &= ((u & 0x007fffffu) == 0u && u >= 0x00800000u && u < 0x7f800000u);
+    }
     g.diag = diag;
+    g.pow2 = pow2;
     g.finite = fin;
     g.zero_rule = zero_rule;
     return g;
@@ -221,7 +229,9 @@ __host__ __device__ __forceinline__ bool flat_voxel(const Grid<ND>& g, const flo
 // window index.  The clipped
 // traversal only evaluates midpoints within a few voxels of the grid (and the
 // grid dimensions are below 2^24 with E < 1), so |c| stays far below 2^22.
-template <int ND>
+//
+// BIASED: c already holds RN(c + 1.5*2^23) (MidIndex::from_ends_biased).
+template <int ND, bool BIASED = false>
 __host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w, const uint32_t* kb, const float* c,
                                                             uint32_t* idx) {
     const float kMagic = 12582912.0f;  // 1.5 * 2^23
@@ -229,14 +239,15 @@ __host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w,
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        const float f = c[a] + kMagic;
+        const float f = BIASED ? c[a] : c[a] + kMagic;
         r[a] = __builtin_bit_cast(uint32_t, f) - kb[a];
         ok = ok && (r[a] < (uint32_t)w.ext[a]);
     }
+    // nested multiply-adds: one full-rate v_mad_u32_u24 (or shift-add) per axis
     if (ND == 3) {
-        *idx = mul_stride(r[0], w.stride[0]) + mul_stride(r[1], w.stride[1]) + r[ND - 1];
+        *idx = mad24(r[0], (uint32_t)w.stride[0], mul_stride(r[1], w.stride[1]) + r[ND - 1]);
     } else {
-        *idx = mul_stride(r[0], w.stride[0]) + r[ND - 1];
+        *idx = mad24(r[0], (uint32_t)w.stride[0], r[ND - 1]);
     }
     return ok;
 }
@@ -431,6 +442,16 @@ struct MidIndex {
 #pragma unroll
         for (int a = 0; a < ND; ++a) b2[a] = twice_exact(g.b[a]);
     }
+    // Diagonal grid whose h[a][a] are powers of two (Grid::pow2): returns
+    // RN(c + 1.5*2^23) for flat_voxel_bounded<ND, true>, as one fma per axis.
+    // h * sh is exact (a power-of-two scaling that neither overflows, since
+    // |c| < 2^22, nor rounds unless |h * sh| < 2^-126), so
+    // fma(h, sh, 1.5*2^23) = RN(h*sh + 1.5*2^23) = RN(RN(h*sh) + 1.5*2^23);
+    // below 2^-126 both sides are 1.5*2^23.
+    __host__ __device__ __forceinline__ void from_ends_biased(const float* p0, const float* p1, float* c) const {
+#pragma unroll
+        for (int a = 0; a < ND; ++a) c[a] = __builtin_fmaf(h[a * ND + a], (p0[a] + p1[a]) - b2[a], 12582912.0f);
+    }
     __host__ __device__ __forceinline__ void from_ends(const float* p0, const float* p1, float* c) const {
         float sh[ND];
 #pragma unroll
@@ -452,20 +473,28 @@ struct MidIndex {
 
 // Visits every segment of one ray whose rounded midpoint lies in the window,
 // in increasing t, as visit(window_flat_index, length).  Zero-length segments
-// are not visited (they contribute exactly zero to both projections).  The
+// are not visited (they contribute exactly zero to both projections) unless
+// ZERO is set.  The
 // window's bounding planes take the place of the grid's in the clipping
 // argument above, so each segment of the full traversal is visited by exactly
 // the window that contains its voxel.
 //
 // PAIR unrolls the segment loop by two (see walk below).
 //
+// POW2 (callers pass it only for a grid with Grid::pow2 set) forms the torch
+// rule's midpoint index with MidIndex::from_ends_biased: the same indices in
+// fewer instructions.  ZERO also visits zero-length segments, for callers
+// whose visit adds an exact zero for them (finite values).
+//
 // e_bound >= 0 replaces the per-ray midpoint error bound E by a caller's bound
 // that holds for this ray on every axis (e.g. the maximum over a view): a
 // larger E only widens [t_lo, t_hi], which produces the same segments.
-template <int ND, int MODE, bool DIAG, bool PAIR = true, class Visit>
+template <int ND, int MODE, bool DIAG, bool PAIR = true, bool POW2 = false, bool ZERO = false, class Visit>
 __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, const Window<ND>& win,
                                                           const float* src, const float* dst, Visit&& visit,
                                                           float e_bound = -1.0f) {
+    static_assert(!POW2 || DIAG, "POW2 grids are diagonal");
+    constexpr bool kFold = POW2 && MODE == MODE_TORCH;
     if (!g.finite) return;  // the reference's indices are all out of range then
     // Non-finite endpoints make every reference world point INF or NaN, and
     // every such segment is masked (3d_torch.py:233-260): no contribution.
@@ -611,7 +640,10 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
                     ss = (a == 0) ? sq : ss + sq;
                 }
                 len = seg_sqrt(ss);
-                mi.from_ends(pa, pb, c);
+                if (kFold)
+                    mi.from_ends_biased(pa, pb, c);
+                else
+                    mi.from_ends(pa, pb, c);
             } else {
                 float m[ND];
                 const float tm = 0.5f * (t0 + t1);
@@ -622,8 +654,8 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
             }
             t0 = t1;
             uint32_t idx;
-            const bool in = flat_voxel_bounded<ND>(win, kb, c, &idx);
-            if (in && len > 0.0f) visit(idx, len);
+            const bool in = flat_voxel_bounded<ND, kFold>(win, kb, c, &idx);
+            if (in && (ZERO || len > 0.0f)) visit(idx, len);
         };
         // PAIR: two segments per trip with the end-point registers swapping
         // roles, so no point is copied (fewer instructions, more registers).
@@ -653,10 +685,10 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
 }
 
 // The whole grid: visit(flat_index, length) with the reference's flat index.
-template <int ND, int MODE, bool DIAG, class Visit>
+template <int ND, int MODE, bool DIAG, bool POW2 = false, class Visit>
 __host__ __device__ __forceinline__ void trace_ray(const Grid<ND>& g, const float* src, const float* dst,
                                                    Visit&& visit) {
-    trace_ray_window<ND, MODE, DIAG>(g, full_window(g), src, dst, visit);
+    trace_ray_window<ND, MODE, DIAG, true, POW2>(g, full_window(g), src, dst, visit);
 }
 
 // Emits, in increasing order, every plane crossing t in [0, 1] of the whole

@@ -6,7 +6,7 @@
 
 using namespace ctlab;
 
-template <int ND, int MODE, bool DIAG>
+template <int ND, int MODE, bool DIAG, bool POW2 = false>
 static int64_t run(const Grid<ND>& g, const float* src, const float* dst, int64_t n_ray, int64_t cap,
                    int64_t* idx, float* len, int64_t* counts) {
     int64_t worst = 0;
@@ -14,7 +14,7 @@ static int64_t run(const Grid<ND>& g, const float* src, const float* dst, int64_
         int64_t k = 0;
         int64_t* io = idx + r * cap;
         float* lo = len + r * cap;
-        trace_ray<ND, MODE, DIAG>(g, src + r * ND, dst + r * ND, [&](int i, float l) {
+        trace_ray<ND, MODE, DIAG, POW2>(g, src + r * ND, dst + r * ND, [&](int i, float l) {
             if (k < cap) { io[k] = i; lo[k] = l; }
             ++k;
         });
@@ -30,6 +30,7 @@ static int64_t dispatch(int mode, int zero_rule, const int* n, const float* minv
                         float* len, int64_t* counts) {
     Grid<ND> g = load_grid<ND>(n, minv, b, zero_rule);
     if (mode == MODE_TORCH) {
+        if (g.pow2) return run<ND, MODE_TORCH, true, true>(g, src, dst, n_ray, cap, idx, len, counts);
         return g.diag ? run<ND, MODE_TORCH, true>(g, src, dst, n_ray, cap, idx, len, counts)
                       : run<ND, MODE_TORCH, false>(g, src, dst, n_ray, cap, idx, len, counts);
     }
@@ -66,14 +67,14 @@ extern "C" int64_t host_trace(int ndim, int mode, int zero_rule, const int* n, c
 
 // Window variant: segments whose voxel lies in [lo, lo+ext), reported with
 // their full-grid flat index.
-template <int ND, int MODE, bool DIAG>
+template <int ND, int MODE, bool DIAG, bool POW2 = false>
 static void run_window(const Grid<ND>& g, const Window<ND>& w, const float* src, const float* dst, int64_t n_ray,
                        int64_t cap, int64_t* idx, float* len, int64_t* counts) {
     for (int64_t r = 0; r < n_ray; ++r) {
         int64_t k = 0;
         int64_t* io = idx + r * cap;
         float* lo = len + r * cap;
-        trace_ray_window<ND, MODE, DIAG>(g, w, src + r * ND, dst + r * ND, [&](uint32_t li, float l) {
+        trace_ray_window<ND, MODE, DIAG, true, POW2>(g, w, src + r * ND, dst + r * ND, [&](uint32_t li, float l) {
             // window index (compact strides) -> grid flat index
             int64_t rem = li, gi = 0;
             int64_t loc[ND];
@@ -99,7 +100,8 @@ extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const in
         for (int a = 0; a < 3; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
         w.stride[2] = 1; w.stride[1] = ext[2]; w.stride[0] = ext[1] * ext[2];
         if (mode == MODE_TORCH) {
-            if (g.diag) run_window<3, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            if (g.pow2) run_window<3, MODE_TORCH, true, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else if (g.diag) run_window<3, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
             else run_window<3, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
         } else {
             if (g.diag) run_window<3, MODE_OTF, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
@@ -111,7 +113,8 @@ extern "C" int64_t host_trace_window(int ndim, int mode, int zero_rule, const in
         for (int a = 0; a < 2; ++a) { w.lo[a] = lo[a]; w.ext[a] = ext[a]; }
         w.stride[1] = 1; w.stride[0] = ext[1];
         if (mode == MODE_TORCH) {
-            if (g.diag) run_window<2, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            if (g.pow2) run_window<2, MODE_TORCH, true, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
+            else if (g.diag) run_window<2, MODE_TORCH, true>(g, w, src, dst, n_ray, cap, idx, len, counts);
             else run_window<2, MODE_TORCH, false>(g, w, src, dst, n_ray, cap, idx, len, counts);
         } else {
             if (g.diag) run_window<2, MODE_OTF, true>(g, w, src, dst, n_ray, cap, idx, len, counts);

@@ -167,6 +167,39 @@ def test_windows_partition_the_traversal(host, host_window, mode, nd):
             assert a == bb, (trial, r)
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+def test_power_of_two_voxel_sizes(host, host_window, nd):
+    """Voxel sizes +-2^k make the midpoint scale exact, and the torch-rule
+    traversal then forms each rounded midpoint with one fma
+    (MidIndex::from_ends_biased).  Index lists and lengths equal the oracle's
+    bit for bit, on random and half-integer-lattice rays (midpoints on voxel
+    faces, where the rounding ties), over the whole grid and over windows."""
+    import itertools
+
+    rng = np.random.default_rng(130 + nd)
+    zr = "exact" if nd == 2 else "thresh"
+    for trial in range(8):
+        shape = tuple(int(x) for x in rng.integers(2, 20, nd))
+        s = (2.0 ** rng.integers(-3, 3, nd) * rng.choice([-1.0, 1.0], nd)).astype(np.float32)
+        minv = np.diag(1.0 / s).astype(np.float32)
+        b = (-(np.array(shape) - 1) / 2 * s + rng.integers(-4, 5, nd) * 0.25).astype(np.float32)
+        if trial % 2 == 0:
+            src, dst = lattice_rays(rng, nd, 4000, lo=-40, hi=40)
+            src, dst = src * np.abs(s), dst * np.abs(s)
+        else:
+            src, dst = random_rays(rng, shape, np.abs(s), 1500, "far" if trial % 4 == 1 else "inside")
+        _check(host, shape, minv, b, src, dst, "torch")
+        full = host(shape, minv, b, src, dst, "torch", zr)
+        parts = [_partition(rng, shape[a], 2) for a in range(nd)]
+        got = [[] for _ in range(len(src))]
+        for combo in itertools.product(*parts):
+            res = host_window(shape, minv, b, [c[0] for c in combo], [c[1] for c in combo], src, dst, "torch", zr)
+            for r in range(len(src)):
+                got[r].extend(zip(res[r][0].tolist(), res[r][1].tolist()))
+        for r in range(len(src)):
+            assert sorted(zip(full[r][0].tolist(), full[r][1].tolist())) == sorted(got[r]), (trial, r)
+
+
 def test_half_and_twice_exact_on_every_float():
     """The traversal's integer forms of 0.5*x and 2*x (used for its midpoint
     constants) equal the FPU products bit for bit on all 2^32 patterns."""
