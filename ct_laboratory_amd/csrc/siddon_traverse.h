@@ -252,6 +252,19 @@ __host__ __device__ __forceinline__ bool flat_voxel_bounded(const Window<ND>& w,
     return ok;
 }
 
+// Running sum of squared coordinate differences of a segment: the torch
+// rule's RN(RN(ss) + RN(d*d)) on the host (test harness, bit for bit with the
+// oracle), one fma on the device (at most one rounding per term either way;
+// lengths only scale contributions, voxel indices never depend on them).
+__host__ __device__ __forceinline__ float add_sq(float ss, float d, bool first) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return first ? d * d : __builtin_fmaf(d, d, ss);
+#else
+    const float sq = d * d;
+    return first ? sq : ss + sq;
+#endif
+}
+
 // Square root of a segment length.  The device uses the hardware
 // v_sqrt_f32 (within 1 ulp); the host build (test harness) uses the correctly
 // rounded sqrtf.  Lengths only scale contributions; voxel indices never depend
@@ -635,9 +648,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
 #pragma unroll
                 for (int a = 0; a < ND; ++a) {
                     pb[a] = src[a] + t1 * vw[a];
-                    const float dd = pb[a] - pa[a];
-                    const float sq = dd * dd;
-                    ss = (a == 0) ? sq : ss + sq;
+                    ss = add_sq(ss, pb[a] - pa[a], a == 0);
                 }
                 len = seg_sqrt(ss);
                 if (kFold)
@@ -834,9 +845,7 @@ __host__ __device__ __forceinline__ float segment_mid(const MidIndex<ND, DIAG>& 
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         p1[a] = src[a] + t1 * vw[a];
-        const float dd = p1[a] - p0[a];
-        const float sq = dd * dd;
-        ss = (a == 0) ? sq : ss + sq;
+        ss = add_sq(ss, p1[a] - p0[a], a == 0);
     }
     const float len = seg_sqrt(ss);
     mi.from_ends(p0, p1, c);

@@ -14,7 +14,7 @@
   256^2 panel, 20 iterations each): alone equal to in the mix, and the first
   iteration's back projection of the residual against the oracle's.
 
-Bar: voxel index lists bit-exact (lengths within 1 ulp); projections within
+Bar: voxel index lists bit-exact (lengths within 3 ulp); projections within
 1e-5 relative (floor 1e-3 of the max); back projections of signed data within
 1e-5 of the back projection of their magnitudes, voxel by voxel.
 """
@@ -65,7 +65,7 @@ def test_c2_segment_lists_sample(dev, cone256):
     oi, ol, oc = O.trace(shape, g["minv"], g["b"], s, d, mode="torch")
     ol_ = positive_lists(oi, ol, oc)
     bad = [r for r in range(len(gl)) if len(gl[r][0]) != len(ol_[r][0]) or (gl[r][0] != ol_[r][0]).any()
-           or (np.abs(gl[r][1].view(np.int32) - ol_[r][1].view(np.int32)) > 1).any()]
+           or (np.abs(gl[r][1].view(np.int32) - ol_[r][1].view(np.int32)) > 3).any()]
     assert not bad, f"{len(bad)} of 50000 rays differ, first {bad[:5]}"
     assert sum(len(x[0]) for x in gl) > 50_000 * 100  # most sampled rays cross the volume
 

