@@ -1402,7 +1402,7 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
                 s[a] = src[3ull * ray + a];
                 d[a] = dst[3ull * ray + a];
             }
-            trace_ray_window<3, MODE, DIAG, false, POW2, ZERO>(
+            trace_ray_window<3, MODE, DIAG, true, POW2, ZERO>(
                 g, win, s, d, [&](uint32_t li, float len) { add(li, val, len); }, e_hint);
         }
     }

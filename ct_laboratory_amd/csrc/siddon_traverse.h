@@ -377,24 +377,33 @@ struct Merge {
     // evaluates its own candidate from its own registers and only the results
     // are selected: selecting the operands instead lets the compiler turn the
     // per-axis state into an indexed stack array (scratch memory).
-    template <bool EXH = true>
+    //
+    // TIES: every axis whose pending crossing equals the popped value
+    // advances at once, so a crossing shared by several axes comes out once
+    // (the segment walks, whose zero-length pieces between equal crossings
+    // contribute nothing).  Otherwise one axis advances per pop and equal
+    // crossings come out once per axis (the crossing lists).
+    template <bool EXH = true, bool TIES = false>
     __host__ __device__ __forceinline__ bool pop(float t_hi, float& t) {
         const float m = (ND == 3) ? fminf(fminf(n0, n1), n2) : fminf(n0, n1);
         if (!(m <= t_hi)) return false;
         const bool k0 = (n0 == m);
-        const bool k1 = !k0 && (n1 == m);
+        const bool k1 = (TIES || !k0) && (n1 == m);
         float g0, g1, g2 = 0.0f;
         const float c0 = advance<EXH>(f0, s0, d0, y0, e0, &g0);
         const float c1 = advance<EXH>(f1, s1, d1, y1, e1, &g1);
         f0 = k0 ? g0 : f0;
         n0 = k0 ? c0 : n0;
         if (ND == 3) {
-            const bool k2 = !k0 && !k1;
+            const bool k2 = TIES ? (n2 == m) : (!k0 && !k1);
             const float c2 = advance<EXH>(f2, s2, d2, y2, e2, &g2);
             f1 = k1 ? g1 : f1;
             n1 = k1 ? c1 : n1;
             f2 = k2 ? g2 : f2;
             n2 = k2 ? c2 : n2;
+        } else if (TIES) {
+            f1 = k1 ? g1 : f1;
+            n1 = k1 ? c1 : n1;
         } else {
             f1 = k0 ? f1 : g1;
             n1 = k0 ? n1 : c1;
@@ -625,13 +634,13 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         float t0;
         if (MODE == MODE_OTF && t_lo == 0.0f) {
             t0 = 0.0f;  // endpoint inserted by collect_hits (3d.cu:890)
-        } else if (!mg.template pop<EXH>(t_stop, t0)) {
+        } else if (!mg.template pop<EXH, true>(t_stop, t0)) {
             return;
         }
         const bool close_at_one = (MODE == MODE_OTF) && (t_stop == 1.0f);
         // Next segment end: the next crossing, or t = 1 closing an otf ray.
         auto next = [&](float& t1, bool& last) -> bool {
-            if (mg.template pop<EXH>(t_stop, t1)) return true;
+            if (mg.template pop<EXH, true>(t_stop, t1)) return true;
             if (close_at_one && t0 < 1.0f) {
                 t1 = 1.0f;  // endpoint inserted by collect_hits (3d.cu:891)
                 last = true;
