@@ -1234,7 +1234,7 @@ static int filed_unit_slots(int64_t n_view) {
     const int64_t half = (n_view + 1) / 2, V = filed_chunk_views(n_view);
     return (int)((half + V - 1) / V) * kBuckets;
 }
-constexpr int kFileWaveQueue = 128;  // per-wave LDS queue of rays with a nonzero value
+constexpr int kFileWaveQueue = 128;  // per-wave LDS queue of (ray, value) with a nonzero value
 // Back projector workgroups: 12 waves, three per SIMD, two workgroups per CU
 // (6 waves per SIMD, 80 VGPRs each).  Wave counts that are not a multiple of
 // four leave the SIMDs unevenly loaded; measured on the 256^3 / 720-view
@@ -1332,7 +1332,10 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
                 const uint32_t l2 = 31u - (uint32_t)__clz(est);
                 const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
                 const int ch = (vs + j - v_lo) / chunk_views;
-                const uint32_t slot = atomicAdd(&bcnt[ch * kBuckets + (kBuckets - 1 - (int)bk)], 1u);
+                // classes longest first in even chunks, shortest first in odd ones: where two
+                // chunks meet in the list, a wave's 64 rays have similar lengths
+                const int cls = (ch & 1) ? (int)bk : kBuckets - 1 - (int)bk;
+                const uint32_t slot = atomicAdd(&bcnt[ch * kBuckets + cls], 1u);
                 if (FILL && (int64_t)slot < n_ids) ids[slot] = ray;
             }
         }
@@ -1353,7 +1356,7 @@ template <int MODE, bool DIAG, bool POW2, bool ZERO, class Add>
 __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& win, const float* __restrict__ sn,
                                             const float* __restrict__ src, const float* __restrict__ dst,
                                             const uint32_t* __restrict__ list, uint32_t n, int64_t n_ray,
-                                            float e_brick, float s1, float s2, uint32_t* next, uint32_t* wq,
+                                            float e_brick, float s1, float s2, uint32_t* next, uint2* wq,
                                             Add&& add) {
     const uint32_t lane = threadIdx.x & 63u;
     const float e_hint = (e_brick <= 8.0f) ? e_brick : -1.0f;
@@ -1369,15 +1372,16 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
             } else {
                 const uint32_t q = base + lane;
                 uint32_t ray = 0u;
-                bool nz = false;
+                float v = 0.0f;
                 if (q < n) {
                     ray = list[q];
-                    nz = (int64_t)ray < n_ray && sn[ray] != 0.0f;  // ids of n_ray or more: no ray
+                    if ((int64_t)ray < n_ray) v = sn[ray];  // ids of n_ray or more: no ray
                 }
+                const bool nz = v != 0.0f;
                 const uint64_t mask = __ballot(nz);
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                if (nz) wq[pend + below] = ray;  // pend + below < 64 + 64
+                if (nz) wq[pend + below] = make_uint2(ray, __float_as_uint(v));  // pend + below < 64 + 64
                 pend += (uint32_t)__popcll(mask);
                 continue;
             }
@@ -1387,16 +1391,17 @@ __device__ __forceinline__ void filed_items(const Grid<3>& g, const Window<3>& w
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t take = min(pend, 64u);
-        const uint32_t ray = wq[lane];
+        const uint2 item = wq[lane];
         const bool keep = 64u + lane < pend;
-        const uint32_t rest = keep ? wq[64u + lane] : 0u;
+        const uint2 rest = keep ? wq[64u + lane] : make_uint2(0u, 0u);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (keep) wq[lane] = rest;
         pend -= take;
         if (lane < take) {
-            const float val = sn[ray] * s1 * s2;
+            const uint32_t ray = item.x;
+            const float val = __uint_as_float(item.y) * s1 * s2;
             float s[3], d[3];
             for (int a = 0; a < 3; ++a) {
                 s[a] = src[3ull * ray + a];
@@ -1422,7 +1427,7 @@ __device__ __forceinline__ void filed_unit_rest(const Grid<3>& g, const Window<3
                                                 const uint32_t* __restrict__ list, uint32_t n, int64_t n_ray,
                                                 float vmax, float e_brick, uint32_t total, int accumulate,
                                                 int split, unsigned long long* acc, uint32_t* s_next,
-                                                uint32_t* my_q) {
+                                                uint2* my_q) {
     const int tid = threadIdx.x;
     const int ey = win.ext[1], ez = win.ext[2];
     const int nvox = win.ext[0] * ey * ez;
@@ -1503,7 +1508,7 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     const int64_t* __restrict__ offsets, int unit_slots, const uint32_t* __restrict__ ids, int accumulate,
     int split, const int32_t* __restrict__ order) {
     __shared__ unsigned long long acc[kBrickLds];
-    __shared__ uint32_t wq[kFiledThreads / 64 * kFileWaveQueue];
+    __shared__ uint2 wq[kFiledThreads / 64 * kFileWaveQueue];
     __shared__ float s_red[kFiledThreads / 64];
     __shared__ float s_red2[kFiledThreads / 64];
     __shared__ uint32_t s_next;
@@ -1803,7 +1808,7 @@ __global__ __launch_bounds__(kFiledThreads) __attribute__((amdgpu_waves_per_eu(6
     const float* __restrict__ minv, const float* __restrict__ b, const float* __restrict__ vmax_in,
     int accumulate, int split, const int32_t* __restrict__ order) {
     __shared__ unsigned long long acc[kBrickLds];
-    __shared__ uint32_t wq[kFiledThreads / 64 * kFileWaveQueue];
+    __shared__ uint2 wq[kFiledThreads / 64 * kFileWaveQueue];
     __shared__ uint32_t s_next;
     const int tid = threadIdx.x;
     const uint32_t ux = order ? (uint32_t)order[blockIdx.x] : blockIdx.x;
