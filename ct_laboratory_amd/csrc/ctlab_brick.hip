@@ -1254,8 +1254,9 @@ __device__ __forceinline__ float filed_brick_e(const float* __restrict__ view_e,
     return block_max(emax, red) * 1.0009765625f;
 }
 
-// Pass 1 (FILL false) counts the rays of every unit per length class into
-// counts[unit * kBuckets + (kBuckets - 1 - class)]; pass 2 (FILL true) writes
+// Pass 1 (FILL false) counts the rays of every unit per view chunk and length
+// class into counts[unit * S + chunk * kBuckets + j], j = kBuckets - 1 - class
+// in even chunks and class in odd ones (S = chunks * kBuckets); pass 2 (FILL true) writes
 // each ray id at offsets[that index] + a cursor, offsets being the exclusive
 // prefix sums of the counts.  One block per unit; items are decoded as
 // brick_items decodes them (plan entries and frames of kStage views staged in
@@ -1776,8 +1777,8 @@ __global__ __launch_bounds__(256) void k_bin_file(const float* __restrict__ src,
             const uint32_t l2 = 31u - (uint32_t)__clz(est);
             bc = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
         }
-        return (int)(ray & 1u) * unit_slots + (int)((int64_t)(ray >> 1) / chunk_rays) * kBuckets +
-               (kBuckets - 1 - (int)bc);
+        const int ch = (int)((int64_t)(ray >> 1) / chunk_rays);
+        return (int)(ray & 1u) * unit_slots + ch * kBuckets + ((ch & 1) ? (int)bc : kBuckets - 1 - (int)bc);
     };
     for (int i = tid; i < nslot; i += 256) hist[i] = 0u;
     __syncthreads();

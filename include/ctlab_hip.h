@@ -391,7 +391,8 @@ int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batch, const fl
  *   4. ctlab_brick_bins_file_3d refiles each brick's list in place order into
  *      filed[offsets[k] .. offsets[k+1]): two units per brick (rays of even and
  *      of odd id), each in chunks of consecutive ids, each chunk grouped by 32
- *      classes of estimated crossing count, longest first; slot_offsets
+ *      classes of estimated crossing count, longest first in even chunks and
+ *      last in odd ones; slot_offsets
  *      [n_slot + 1] (int64, n_slot from ctlab_brick_bins_slots_3d) holds the
  *      start of every slot, slot (brick * 2 + parity) * S + chunk * 32 + j with
  *      S = n_slot / (2 * n_brick).
