@@ -546,10 +546,6 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
     const float kInf = __builtin_huge_valf();
     float e_max = 0.0f, t_cap = kInf;  // t_cap: first crossing of a plane beyond the window
 
-    // Every axis is evaluated in full and its results are kept by selects (no
-    // branches: their exec-mask merges cost more register copies than the
-    // few discarded values of an inactive axis).
-    bool dead = false;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         dir[a] = dv[a] - sv[a];
@@ -565,7 +561,7 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         // Midpoint coordinates stay within n + 5E of the grid; with E <= 8 they
         // are far below the 2^22 bound of flat_voxel_bounded.  Endpoints that far
         // out (~1e11 voxels) carry no usable float32 crossings anyway.
-        dead = dead || !(E <= 8.0f);
+        if (!(E <= 8.0f)) return;
         e_max = fmaxf(e_max, E);
         bool active;
         if (MODE == MODE_OTF) {
@@ -573,37 +569,41 @@ __host__ __device__ __forceinline__ void trace_ray_window(const Grid<ND>& g, con
         } else {
             active = (g.zero_rule == ZERO_EXACT) ? (dir[a] != 0.0f) : !(fabsf(dir[a]) < 1.0e-12f);
         }
-        // Constant coordinate: a ray outside the slab by more than E has no
-        // in-range segment at all.
-        dead = dead || (!active && (sv[a] < (float)win.lo[a] - 0.5f - E ||
-                                    sv[a] > (float)(win.lo[a] + win.ext[a]) - 0.5f + E));
-        const float ya = 1.0f / dir[a];
+        step[a] = 0;
+        y[a] = 1.0f;
+        if (!active) {
+            // Constant coordinate: a ray outside the slab by more than E has no
+            // in-range segment at all.
+            if (sv[a] < (float)win.lo[a] - 0.5f - E || sv[a] > (float)(win.lo[a] + win.ext[a]) - 0.5f + E) return;
+            continue;
+        }
+        y[a] = 1.0f / dir[a];
         const bool up = dir[a] > 0.0f;
+        step[a] = up ? 1 : -1;
         const int plo = win.lo[a], phi = win.lo[a] + win.ext[a];
-        const float t_in = crossing_p<MODE>(up ? plo : phi, sv[a], dir[a], ya);
-        const float t_out = crossing_p<MODE>(up ? phi : plo, sv[a], dir[a], ya);
-        const float t_c = crossing_p<MODE>(up ? phi + 1 : plo - 1, sv[a], dir[a], ya);
+        const float t_in = crossing_p<MODE>(up ? plo : phi, sv[a], dir[a], y[a]);
+        const float t_out = crossing_p<MODE>(up ? phi : plo, sv[a], dir[a], y[a]);
         // 4E/|dir| from y = RN(1/dir): within 2^-23 relative, far inside the
         // factor-of-two slack of the clipping argument.
-        const float tau = 4.0f * E * fabsf(ya) + 9.5367431640625e-07f;  // + 2^-20
-        y[a] = active ? ya : 1.0f;
-        step[a] = active ? (up ? 1 : -1) : 0;
-        t_lo = active ? fmaxf(t_lo, t_in - tau) : t_lo;
-        t_hi = active ? fminf(t_hi, t_out + tau) : t_hi;
-        t_cap = active ? fminf(t_cap, t_c) : t_cap;
+        const float tau = 4.0f * E * fabsf(y[a]) + 9.5367431640625e-07f;  // + 2^-20
+        t_lo = fmaxf(t_lo, t_in - tau);
+        t_hi = fminf(t_hi, t_out + tau);
+        t_cap = fminf(t_cap, crossing_p<MODE>(up ? phi + 1 : plo - 1, sv[a], dir[a], y[a]));
     }
-    if (dead || !(t_lo <= t_hi)) return;
+    if (!(t_lo <= t_hi)) return;
 
     // Jump each axis to its first crossing >= t_lo.
     Merge<ND, MODE> mg;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        // an axis without crossings (step 0) runs the bounded search on its
-        // placeholder values and keeps no pending crossing (INF)
+        if (step[a] == 0) continue;
         const int plo = win.lo[a], phi = win.lo[a] + win.ext[a];
         const int p = first_plane<MODE>(sv[a], dir[a], y[a], plo, phi, step[a], t_lo);
-        const float tp = crossing_p<MODE>(p, sv[a], dir[a], y[a]);
-        const float t = (step[a] != 0 && p >= plo && p <= phi && tp <= 1.0f) ? tp : kInf;
+        float t = kInf;
+        if (p >= plo && p <= phi) {
+            t = crossing_p<MODE>(p, sv[a], dir[a], y[a]);
+            if (t > 1.0f) t = kInf;
+        }
         mg.set(a, sv[a], dir[a], y[a], p, step[a], plo, phi, t);
     }
 
