@@ -16,9 +16,8 @@
 // brick first takes the largest |sinogram| value over its rays; with the
 // longest possible segment (the voxel's body diagonal) and the number of rays
 // this bounds every voxel's sum, and a power-of-two scale maps that bound just
-// below 2^62.  Each contribution sino*len (the reference's product) is scaled
-// exactly and its exact value rounded to an integer (to_fixed_prod, one fma in
-// double): the quantum is 2^-62 of the bound,
+// below 2^62.  Each float contribution sino*len (the reference's product) is
+// scaled exactly and rounded to an integer: the quantum is 2^-62 of the bound,
 // far below float accumulation error, and integer addition makes the result
 // independent of the order of the rays (deterministic).  Bricks whose rays
 // carry non-finite values fall back to float atomics into global memory so
@@ -615,7 +614,7 @@ __device__ __forceinline__ void brick_unit(
     // most the voxel diagonal, so |sum| <= total * vmax * diag per voxel; the
     // factor 2 covers the rounding of len and of the product.  2^F maps that
     // bound below 2^62 (no overflow of the signed 64-bit sums) and each single
-    // contribution below 2^50 (the range of the conversion in to_fixed_prod).  It is
+    // contribution below 2^50 (the range of the conversion in to_fixed).  It is
     // applied to each sinogram value as two float factors, so any F in
     // [-250, 250] is exact.  The absolute error of a voxel is at most half a
     // quantum 2^-F per contribution.  Non-finite data, or products that could
@@ -635,7 +634,7 @@ __device__ __forceinline__ void brick_unit(
     // 3. trace and accumulate
     if (vmax != 0.0f) {
         if (fixed) {
-            auto add = [&](uint32_t li, float v, float len) { atomicAdd(&acc[li], to_fixed_prod(v, len)); };
+            auto add = [&](uint32_t li, float v, float len) { atomicAdd(&acc[li], to_fixed(v * len)); };
             if (g.diag)
                 brick_items<MODE, true>(g, win, sn, src, dst, p, frames, n_view, rows, cols, item_lo, item_hi, total, e_brick, s1,
                                         s2, queue, bcnt, s_plan, s_frame, add);
@@ -1445,7 +1444,7 @@ __device__ __forceinline__ void filed_unit_rest(const Grid<3>& g, const Window<3
     if (vmax != 0.0f) {
         if (fixed) {
             // finite values: a zero-length segment adds an exact zero, so it is not tested for
-            auto add = [&](uint32_t li, float v, float len) { atomicAdd(&acc[li], to_fixed_prod(v, len)); };
+            auto add = [&](uint32_t li, float v, float len) { atomicAdd(&acc[li], to_fixed(v * len)); };
             if (MODE == MODE_TORCH && g.pow2)
                 filed_items<MODE, true, MODE == MODE_TORCH, true>(g, win, sn, src, dst, list, n, n_ray, e_brick,
                                                                    s1, s2, s_next, my_q, add);

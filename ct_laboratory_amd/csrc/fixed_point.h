@@ -18,14 +18,6 @@ __device__ __forceinline__ unsigned long long to_fixed(float c) {
     return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
 }
 
-// The same rounding of the exact product v * len (|v * len| < 2^50): a product
-// of two floats is exact in double, so one fma forms it and adds 1.5 * 2^52
-// with a single rounding.
-__device__ __forceinline__ unsigned long long to_fixed_prod(float v, float len) {
-    const double y = __builtin_fma((double)len, (double)v, 6755399441055744.0);  // 1.5 * 2^52
-    return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
-}
-
 // *p += v with the shader's IEEE single addition (compare-and-swap loop; each
 // failed swap means another writer's add landed, so the loop ends).
 __device__ __forceinline__ void add_f32_ieee(float* p, float v) {
