@@ -1216,6 +1216,7 @@ namespace ctlab {
 
 constexpr int kFileThreads = 256;
 constexpr int kFileHalves = 2;     // units per brick: the first and the second half of the views
+constexpr int kFileBuckets = 64;    // eighth-octave length classes of the filed unit lists
 constexpr int kFileMaxChunks = 64;  // view chunks per unit, at most
 
 // A unit's list is cut into chunks of consecutive views (at least 8 views, at
@@ -1231,7 +1232,7 @@ static int filed_chunk_views(int64_t n_view) {
 
 static int filed_unit_slots(int64_t n_view) {
     const int64_t half = (n_view + 1) / 2, V = filed_chunk_views(n_view);
-    return (int)((half + V - 1) / V) * kBuckets;
+    return (int)((half + V - 1) / V) * kFileBuckets;
 }
 constexpr int kFileWaveQueue = 128;  // per-wave LDS queue of (ray, value) with a nonzero value
 // Back projector workgroups: 12 waves, three per SIMD, two workgroups per CU
@@ -1254,8 +1255,8 @@ __device__ __forceinline__ float filed_brick_e(const float* __restrict__ view_e,
 }
 
 // Pass 1 (FILL false) counts the rays of every unit per view chunk and length
-// class into counts[unit * S + chunk * kBuckets + j], j = kBuckets - 1 - class
-// in even chunks and class in odd ones (S = chunks * kBuckets); pass 2 (FILL true) writes
+// class into counts[unit * S + chunk * kFileBuckets + j], j = kFileBuckets - 1 -
+// class in even chunks and class in odd ones (S = chunks * kFileBuckets); pass 2 (FILL true) writes
 // each ray id at offsets[that index] + a cursor, offsets being the exclusive
 // prefix sums of the counts.  One block per unit; items are decoded as
 // brick_items decodes them (plan entries and frames of kStage views staged in
@@ -1267,7 +1268,7 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
     const float* __restrict__ view_e, const float* __restrict__ frames, const float* __restrict__ minv,
     const float* __restrict__ b, int chunk_views, int unit_slots, uint32_t* __restrict__ counts,
     const int64_t* __restrict__ offsets, uint32_t* __restrict__ ids, int64_t n_ids) {
-    __shared__ uint32_t bcnt[kFileMaxChunks * kBuckets];
+    __shared__ uint32_t bcnt[kFileMaxChunks * kFileBuckets];
     __shared__ int4 s_plan[kStage + 1];
     __shared__ float s_frame[kStage * kFrameFloats];
     __shared__ float red[kFileThreads / 64];
@@ -1330,12 +1331,12 @@ __global__ __launch_bounds__(kFileThreads) void k_file_units(
                                      : segment_estimate<MODE_TORCH>(g, src + 3ull * ray, dst + 3ull * ray, blo, bhi);
             if (est) {
                 const uint32_t l2 = 31u - (uint32_t)__clz(est);
-                const uint32_t bk = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
+                const uint32_t bk = min(8u * l2 + ((est << 3) >> l2) - 8u, (uint32_t)(kFileBuckets - 1));
                 const int ch = (vs + j - v_lo) / chunk_views;
                 // classes longest first in even chunks, shortest first in odd ones: where two
                 // chunks meet in the list, a wave's 64 rays have similar lengths
-                const int cls = (ch & 1) ? (int)bk : kBuckets - 1 - (int)bk;
-                const uint32_t slot = atomicAdd(&bcnt[ch * kBuckets + cls], 1u);
+                const int cls = (ch & 1) ? (int)bk : kFileBuckets - 1 - (int)bk;
+                const uint32_t slot = atomicAdd(&bcnt[ch * kFileBuckets + cls], 1u);
                 if (FILL && (int64_t)slot < n_ids) ids[slot] = ray;
             }
         }

@@ -341,10 +341,10 @@ int ctlab_back_project_3d_brick(const float* sino, int64_t batch, const float* s
  * [0, n_view/2) and [n_view/2, n_view)); its list holds the rays of its
  * shadow rectangles that pass the brick kernel's slab test, in chunks of V
  * consecutive views (V = max(8, ceil(ceil(n_view/2) / 64))), each chunk's
- * rays grouped into 32 classes by estimated crossing count, longest class
- * first in even chunks and last in odd ones.  With
+ * rays grouped into 64 eighth-octave classes by estimated crossing count,
+ * longest class first in even chunks and last in odd ones.  With
  * S = ctlab_brick_filed_slots_3d(...) / (2 * n_brick) slots per unit, slot
- * (unit * S + c * 32 + j) holds class 31 - j of view chunk c (class j when c
+ * (unit * S + c * 64 + j) holds class 63 - j of view chunk c (class j when c
  * is odd) of unit = brick * 2 + half:
  *   1. ctlab_brick_filed_count_3d writes counts[n_slot] (n_slot from
  *      ctlab_brick_filed_slots_3d);

@@ -129,7 +129,8 @@ def test_filed_lists_cover_the_traced_rays(dev):
     nbk = int(N.load().ctlab_brick_count_3d(*shape))
     n_view = n_ray // per_view
     S = int(N.load().ctlab_brick_filed_slots_3d(*shape, n_view)) // (2 * nbk)  # slots per unit
-    assert S % 32 == 0 and off.shape[0] == nbk * 2 * S + 1 and off[-1] <= ids.shape[0]
+    K = 64  # length classes per view chunk
+    assert S % K == 0 and off.shape[0] == nbk * 2 * S + 1 and off[-1] <= ids.shape[0]
     covered = {}
     for bk in range(nbk):
         for part in range(2):
@@ -139,7 +140,7 @@ def test_filed_lists_cover_the_traced_rays(dev):
             views = rays // per_view
             assert np.all((views >= part * n_view // 2) & (views < (part + 1) * n_view // 2))
             # view chunks in order: the views of a unit's list never decrease from chunk to chunk
-            chunk_first = [ids[off[u + c]:off[u + c + 32]] // per_view for c in range(0, S, 32)]
+            chunk_first = [ids[off[u + c]:off[u + c + K]] // per_view for c in range(0, S, K)]
             chunk_views = [(v.min(), v.max()) for v in chunk_first if v.size]
             assert all(a[1] < b[0] for a, b in zip(chunk_views, chunk_views[1:]))
             covered.setdefault(bk, set()).update(rays.tolist())
