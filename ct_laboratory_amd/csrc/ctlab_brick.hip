@@ -1697,13 +1697,14 @@ extern "C" int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batc
 // way the plan's lists are: a brick's list becomes two units, the rays of even
 // and of odd id (about the same work each), each cut into chunks of
 // consecutive ray ids (ray sets in spatial order keep neighbouring rays
-// together) and each chunk grouped by the 32 length classes of the rays' slab
+// together) and each chunk grouped by the 64 length classes of the rays' slab
 // estimates, longest first.  k_back_filed_bins then traces them like
 // k_back_filed.
 
 namespace ctlab {
 
 constexpr int kBinChunks = 64;  // id chunks per unit
+constexpr int kBinBuckets = 64;  // eighth-octave length classes of the brick bins
 
 static int64_t bin_chunk_rays(int64_t n_ray) {
     const int64_t half = (n_ray + 1) / 2;
@@ -1712,7 +1713,7 @@ static int64_t bin_chunk_rays(int64_t n_ray) {
 
 static int bin_unit_slots(int64_t n_ray) {
     const int64_t half = (n_ray + 1) / 2, R = bin_chunk_rays(n_ray);
-    return (int)std::max<int64_t>(1, (half + R - 1) / R) * kBuckets;
+    return (int)std::max<int64_t>(1, (half + R - 1) / R) * kBinBuckets;
 }
 
 // Exclusive scan of v[0, n) in LDS (n <= 16 * 256) by a 256-thread block;
@@ -1753,7 +1754,7 @@ __global__ __launch_bounds__(256) void k_bin_file(const float* __restrict__ src,
                                                   const uint32_t* __restrict__ ids, const float* __restrict__ emax_in,
                                                   int64_t chunk_rays, int unit_slots,
                                                   int64_t* __restrict__ slot_offsets, uint32_t* __restrict__ filed) {
-    __shared__ uint32_t hist[2 * kBinChunks * kBuckets];
+    __shared__ uint32_t hist[2 * kBinChunks * kBinBuckets];
     __shared__ uint32_t wsum[4];
     const int tid = threadIdx.x;
     const int bk = blockIdx.x, nslot = 2 * unit_slots;
@@ -1775,10 +1776,10 @@ __global__ __launch_bounds__(256) void k_bin_file(const float* __restrict__ src,
         uint32_t bc = 0;
         if (est) {
             const uint32_t l2 = 31u - (uint32_t)__clz(est);
-            bc = min(4u * l2 + ((est << 2) >> l2) - 4u, (uint32_t)(kBuckets - 1));
+            bc = min(8u * l2 + ((est << 3) >> l2) - 8u, (uint32_t)(kBinBuckets - 1));
         }
         const int ch = (int)((int64_t)(ray >> 1) / chunk_rays);
-        return (int)(ray & 1u) * unit_slots + ch * kBuckets + ((ch & 1) ? (int)bc : kBuckets - 1 - (int)bc);
+        return (int)(ray & 1u) * unit_slots + ch * kBinBuckets + ((ch & 1) ? (int)bc : kBinBuckets - 1 - (int)bc);
     };
     for (int i = tid; i < nslot; i += 256) hist[i] = 0u;
     __syncthreads();

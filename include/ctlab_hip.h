@@ -390,11 +390,11 @@ int ctlab_back_project_3d_brick_filed(const float* sino, int64_t batch, const fl
  *      first: ids of n_ray or more are no ray);
  *   4. ctlab_brick_bins_file_3d refiles each brick's list in place order into
  *      filed[offsets[k] .. offsets[k+1]): two units per brick (rays of even and
- *      of odd id), each in chunks of consecutive ids, each chunk grouped by 32
- *      classes of estimated crossing count, longest first in even chunks and
- *      last in odd ones; slot_offsets
+ *      of odd id), each in chunks of consecutive ids, each chunk grouped by 64
+ *      eighth-octave classes of estimated crossing count, longest first in
+ *      even chunks and last in odd ones; slot_offsets
  *      [n_slot + 1] (int64, n_slot from ctlab_brick_bins_slots_3d) holds the
- *      start of every slot, slot (brick * 2 + parity) * S + chunk * 32 + j with
+ *      start of every slot, slot (brick * 2 + parity) * S + chunk * 64 + j with
  *      S = n_slot / (2 * n_brick).
  * The lists depend only on the rays, grid, map and rule: build them once per ray
  * set.  ctlab_back_project_3d_brick_bins then computes the same segments as
